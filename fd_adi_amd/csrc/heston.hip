@@ -1,0 +1,1060 @@
+// Fused, batched Heston ADI engine.
+//
+// Replaces the per-step operator-call loops of
+// FiniteDifferenceEngineGPU.pyx:379-593 (and the CPU-only Craig-Sneyd loops
+// of FiniteDifferenceEngine.pyx:778-898) with one launch per ADI stage for a
+// whole batch of options with uneven grids.
+//
+// Layout.  Option k's state is an (nS, nv) row-major block at d_state+off[k]
+// (variance contiguous, as the reference's Grid).  No 2-D operator is stored:
+// every band of A0 (spot), A1 (variance) and A01 (cross) is generated in the
+// kernels from 1-D tables built on the device from the meshes:
+//   A0[i,j] band = P(i) + v_j Q(i)                 (mu_S d/dS + 1/2 v S^2 d2/dS2 - r/2)
+//   A1[i,j] band = [S_i != 0] G(j) - r/2 on mid    (kappa(theta-v) d/dv + 1/2 sigma^2 v d2/dv2)
+//   A01[i,j]     = rho sigma S_i v_j ws(i) x wv(j) (9-point)
+// Boundary rows are those of _TriBandedOperatorGPU.cu:203-362: Dirichlet 0
+// at S=0 (and at Smax for a barrier), dU/dS=1 at Smax, one-sided first
+// derivative at both variance ends, d2/dv2 Von Neumann 0 at v=0 and the
+// second-order backward row (with its -2 band entry, folded) at vmax.
+//
+// Sweeps.  A spot sweep is one workgroup per (option, tile of NL variance
+// columns); a variance sweep one workgroup per (option, tile of NL spot
+// rows).  The tile's right-hand side is formed (fusing every explicit term of
+// the stage) straight into LDS, the NL tridiagonal lines are solved in LDS,
+// and the stage output (fusing the next stage's explicit term) is written
+// back: one read of each input vector and one write per stage, no
+// transposes.  The line solve uses the time-invariant LU factorisation of
+// I - theta dt A (built once per configure): each line is split into chunks
+// owned by threads, and both triangular sweeps are first-order linear
+// recurrences composed across chunks with a wave-level scan of affine maps.
+#include "common.h"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace fdadi {
+namespace {
+
+enum SpotField {
+  SP_PSUB, SP_PMID, SP_PSUP, SP_QSUB, SP_QMID, SP_QSUP, SP_PR, SP_QR,
+  SP_WSUB, SP_WMID, SP_WSUP, SP_S, SP_DIR, SP_NF
+};
+enum VarField {
+  VT_GSUB, VT_GMID, VT_GSUP, VT_WSUB, VT_WMID, VT_WSUP, VT_V, VT_M, VT_RD, VT_N, VT_NF
+};
+
+struct HOpt {
+  int64_t nS, nv, off;  // grid and state offset
+  int64_t so, vo;       // offsets into the spot / variance tables
+  double r, kappa, theta, sigma, rho, dt;
+  double tdt;           // weight of the implicit part (theta*dt, or dt for implicit)
+  double fold_f;        // bottom fold factor of I - tdt A1 (lines with S != 0)
+  double rdS0;          // 1 / (1 + tdt r/2): the S = 0 variance line
+  double Gz;            // A1[nv-1, nv-3] for S != 0
+  int32_t barrier, pad;
+};
+
+struct Tile {
+  int32_t opt;
+  int32_t start;  // first column (spot sweep) / first row (variance sweep)
+};
+
+enum StageMode {
+  // spot sweeps
+  SM_FIRST = 0,  // rhs = U + F U - Le0 U;               out = x - Le1 U
+  SM_HV = 1,     // rhs = U + F U + 1/2 F^(Y2-U) - Le0 Y2; out = x - Le1 Y2
+  SM_CS = 2,     // rhs = U + F U + dt/2 A01 (Y2-U) - Le0 U; out = x - Le1 U
+  SM_MCS = 3,    // rhs = U + F U + th dt A01 D + (1/2-th) F D - Le0 U; out = x - Le1 U
+  SM_IMPL = 4,   // rhs = W + dt R0;                     out = x
+  // variance sweeps
+  VM_PLAIN = 10, // rhs = W;                             out = x
+  VM_IMPL = 11   // rhs = U + dt A01 U;                  out = x
+};
+
+struct Ctx {
+  const HOpt *opts;
+  const double *sp;  // spot tables, field-major, stride sp_n
+  const double *vt;  // variance tables, field-major, stride vt_n
+  int64_t sp_n, vt_n;
+  const double *rd0;  // spot-line pivots 1/u, state layout
+};
+
+__device__ __forceinline__ double SP(const Ctx &c, const HOpt &o, int f, int64_t i) {
+  return c.sp[f * c.sp_n + o.so + i];
+}
+__device__ __forceinline__ double VT(const Ctx &c, const HOpt &o, int f, int64_t j) {
+  return c.vt[f * c.vt_n + o.vo + j];
+}
+
+// (A0 x)(i,j) without residual; 0 on Dirichlet rows (their value is 0 and
+// the Dirichlet row of dt*A0 / theta*dt*A0 is the identity row).  Neighbour
+// values on Dirichlet rows are forced to 0 as apply does.
+template <class F>
+__device__ double a0_dot(const Ctx &c, const HOpt &o, int64_t i, int64_t j, double v, F x) {
+  if (SP(c, o, SP_DIR, i) != 0.0) return 0.0;
+  const double xm = (i > 0 && SP(c, o, SP_DIR, i - 1) == 0.0) ? x(i - 1, j) : 0.0;
+  const double xp = (i < o.nS - 1 && SP(c, o, SP_DIR, i + 1) == 0.0) ? x(i + 1, j) : 0.0;
+  const double sub = SP(c, o, SP_PSUB, i) + v * SP(c, o, SP_QSUB, i);
+  const double mid = SP(c, o, SP_PMID, i) + v * SP(c, o, SP_QMID, i);
+  const double sup = SP(c, o, SP_PSUP, i) + v * SP(c, o, SP_QSUP, i);
+  return sub * xm + mid * x(i, j) + sup * xp;
+}
+
+__device__ __forceinline__ double r0_at(const Ctx &c, const HOpt &o, int64_t i, double v) {
+  return SP(c, o, SP_PR, i) + v * SP(c, o, SP_QR, i);
+}
+
+// (A1 x)(i,j), including the -2 band entry of the last variance row.
+template <class F>
+__device__ double a1_dot(const Ctx &c, const HOpt &o, int64_t i, int64_t j, F x) {
+  const double snz = SP(c, o, SP_S, i) != 0.0 ? 1.0 : 0.0;
+  const double mid = snz * VT(c, o, VT_GMID, j) - 0.5 * o.r;
+  double acc = mid * x(i, j);
+  if (snz != 0.0) {
+    if (j > 0) acc += VT(c, o, VT_GSUB, j) * x(i, j - 1);
+    if (j < o.nv - 1) acc += VT(c, o, VT_GSUP, j) * x(i, j + 1);
+    if (j == o.nv - 1) acc += o.Gz * x(i, j - 2);
+  }
+  return acc;
+}
+
+// (A01 x)(i,j): rho sigma S v times the 9-point central product stencil.
+template <class F>
+__device__ double a01_dot(const Ctx &c, const HOpt &o, int64_t i, int64_t j, F x) {
+  if (o.rho == 0.0 || i == 0 || i == o.nS - 1 || j == 0 || j == o.nv - 1) return 0.0;
+  const double ws[3] = {SP(c, o, SP_WSUB, i), SP(c, o, SP_WMID, i), SP(c, o, SP_WSUP, i)};
+  const double wv[3] = {VT(c, o, VT_WSUB, j), VT(c, o, VT_WMID, j), VT(c, o, VT_WSUP, j)};
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double row = wv[0] * x(i + a - 1, j - 1) + wv[1] * x(i + a - 1, j) + wv[2] * x(i + a - 1, j + 1);
+    acc += ws[a] * row;
+  }
+  const double coef = ((o.rho * o.sigma) * SP(c, o, SP_S, i)) * VT(c, o, VT_V, j);
+  return coef * acc;
+}
+
+// Right-hand side of a spot-sweep stage at (i, j).
+template <int MODE>
+__device__ double spot_rhs(const Ctx &c, const HOpt &o, int64_t i, int64_t j,
+                           const double *__restrict__ U, const double *__restrict__ Y) {
+  const double *u = U + o.off;
+  const int64_t nv = o.nv;
+  auto xu = [&](int64_t a, int64_t b) { return u[a * nv + b]; };
+  const double v = VT(c, o, VT_V, j);
+  if (MODE == SM_IMPL) return xu(i, j) + o.dt * r0_at(c, o, i, v);
+  const double dt = o.dt;
+  const double a0u = a0_dot(c, o, i, j, v, xu);
+  const double r0 = SP(c, o, SP_DIR, i) != 0.0 ? 0.0 : r0_at(c, o, i, v);
+  const double fu = dt * (a0u + r0) + dt * a1_dot(c, o, i, j, xu) + dt * a01_dot(c, o, i, j, xu);
+  double rhs = xu(i, j) + fu;
+  if (MODE == SM_FIRST) return rhs - o.tdt * a0u;
+  const double *y = Y + o.off;
+  auto xy = [&](int64_t a, int64_t b) { return y[a * nv + b]; };
+  auto xd = [&](int64_t a, int64_t b) { return y[a * nv + b] - u[a * nv + b]; };
+  if (MODE == SM_HV) {
+    const double fd = dt * a0_dot(c, o, i, j, v, xd) + dt * a1_dot(c, o, i, j, xd) +
+                      dt * a01_dot(c, o, i, j, xd);
+    return rhs + 0.5 * fd - o.tdt * a0_dot(c, o, i, j, v, xy);
+  }
+  if (MODE == SM_CS) return rhs + (0.5 * dt) * a01_dot(c, o, i, j, xd) - o.tdt * a0u;
+  // SM_MCS: theta = tdt / dt
+  const double th = o.tdt / dt;
+  const double c01 = a01_dot(c, o, i, j, xd);
+  const double fd = dt * (a0_dot(c, o, i, j, v, xd) + r0) + dt * a1_dot(c, o, i, j, xd) + dt * c01;
+  return rhs + (th * dt) * c01 + (0.5 - th) * fd - o.tdt * a0u;
+}
+
+// Output of a spot-sweep stage at (i, j) given the solved value x.
+template <int MODE>
+__device__ double spot_out(const Ctx &c, const HOpt &o, int64_t i, int64_t j, double x,
+                           const double *__restrict__ U, const double *__restrict__ Y) {
+  if (MODE == SM_IMPL) return x;
+  const double *f = (MODE == SM_HV ? Y : U) + o.off;
+  const int64_t nv = o.nv;
+  auto xf = [&](int64_t a, int64_t b) { return f[a * nv + b]; };
+  return x - o.tdt * a1_dot(c, o, i, j, xf);
+}
+
+// Coefficients of the tridiagonal I - tdt*A0 on spot line j at row i.
+__device__ __forceinline__ void li0_row(const Ctx &c, const HOpt &o, int64_t i, double v,
+                                        double &a, double &b, double &cc) {
+  if (SP(c, o, SP_DIR, i) != 0.0) {
+    a = 0.0; b = 1.0; cc = 0.0;
+    return;
+  }
+  a = -o.tdt * (SP(c, o, SP_PSUB, i) + v * SP(c, o, SP_QSUB, i));
+  b = 1.0 - o.tdt * (SP(c, o, SP_PMID, i) + v * SP(c, o, SP_QMID, i));
+  cc = -o.tdt * (SP(c, o, SP_PSUP, i) + v * SP(c, o, SP_QSUP, i));
+  if (i == 0) a = 0.0;
+  if (i == o.nS - 1) cc = 0.0;
+}
+
+// Exclusive scan of affine maps y -> E + M y along the TPL chunks of each of
+// the NL lines of a tile.  Entries live in LDS at [c*NL + l].  Forward
+// (REV=false): carry(0)=0, carry(c+1)=E[c]+M[c]*carry(c).  Backward
+// (REV=true): the same from the last chunk down.  One wave per line.
+template <int THREADS, int NL, bool REV>
+__device__ void affine_scan(const double *cM, const double *cE, double *cIn, int tpl) {
+  constexpr int W = 64;
+  const int lane = threadIdx.x % W, wave = threadIdx.x / W;
+  constexpr int NW = THREADS / W;
+  const int K = (tpl + W - 1) / W;  // entries per lane
+  for (int l = wave; l < NL; l += NW) {
+    // Compose this lane's K entries (in scan order).
+    double Mloc = 1.0, Eloc = 0.0;
+    for (int q = 0; q < K; ++q) {
+      const int s = lane * K + q;
+      if (s < tpl) {
+        const int cidx = REV ? tpl - 1 - s : s;
+        const double m = cM[cidx * NL + l], e = cE[cidx * NL + l];
+        Eloc = e + m * Eloc;
+        Mloc = m * Mloc;
+      }
+    }
+    // Inclusive wave scan of (M, E) (later maps applied after earlier ones).
+    double M = Mloc, E = Eloc;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+      const double Mo = __shfl_up(M, d, W), Eo = __shfl_up(E, d, W);
+      if (lane >= d) {
+        E = E + M * Eo;
+        M = M * Mo;
+      }
+    }
+    // Exclusive: the carry entering this lane's first entry.
+    double carry = __shfl_up(E, 1, W);
+    if (lane == 0) carry = 0.0;
+    for (int q = 0; q < K; ++q) {
+      const int s = lane * K + q;
+      if (s < tpl) {
+        const int cidx = REV ? tpl - 1 - s : s;
+        cIn[cidx * NL + l] = carry;
+        carry = cE[cidx * NL + l] + cM[cidx * NL + l] * carry;
+      }
+    }
+  }
+}
+
+// ---- spot sweep ---------------------------------------------------------------
+// LDS: d[nSmax * NL] (row i, column jj at i*NL+jj) + 3 * TPL * NL carries.
+template <int MODE, int THREADS, int NL, int MAXL>
+__global__ void __launch_bounds__(THREADS)
+spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ U,
+                  const double *__restrict__ Y, double *__restrict__ out, int64_t nSmax) {
+  extern __shared__ double lds[];
+  constexpr int TPL = THREADS / NL;
+  double *d = lds;
+  double *cM = lds + nSmax * NL;
+  double *cE = cM + TPL * NL;
+  double *cIn = cE + TPL * NL;
+  const Tile t = tiles[blockIdx.x];
+  const HOpt o = c.opts[t.opt];
+  const int64_t nS = o.nS, nv = o.nv, j0 = t.start;
+  const int tid = threadIdx.x;
+
+  // Phase 1: fused right-hand side into LDS.
+  for (int64_t q = tid; q < nS * NL; q += THREADS) {
+    const int64_t i = q / NL, j = j0 + q % NL;
+    // Dirichlet rows: the solve forces the right-hand side to the value 0.
+    d[q] = (j < nv && SP(c, o, SP_DIR, i) == 0.0) ? spot_rhs<MODE>(c, o, i, j, U, Y) : 0.0;
+  }
+  __syncthreads();
+
+  // Phase 2: chunked LU solve of (I - tdt A0) x = rhs on each column.
+  const int jj = tid % NL, ch = tid / NL;
+  const int64_t j = j0 + jj;
+  const int L = (int)((nS + TPL - 1) / TPL);
+  const int64_t s = (int64_t)ch * L;
+  const int64_t e = s + L < nS ? s + L : nS;
+  const bool active = j < nv && s < e;
+  const double v = j < nv ? VT(c, o, VT_V, j) : 0.0;
+  const double *rd = c.rd0 + o.off;
+  double mreg[MAXL], rdreg[MAXL];
+  double M = 1.0, E = 0.0;
+  if (active) {
+    double rdprev = s > 0 ? rd[(s - 1) * nv + j] : 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+      const int64_t i = s + k;
+      if (i < e) {
+        double a, b, cc;
+        li0_row(c, o, i, v, a, b, cc);
+        const double m = -a * rdprev;
+        const double rdi = rd[i * nv + j];
+        mreg[k] = m;
+        rdreg[k] = rdi;
+        E = d[i * NL + jj] + m * E;
+        M = m * M;
+        rdprev = rdi;
+      }
+    }
+  }
+  cM[ch * NL + jj] = active ? M : 1.0;
+  cE[ch * NL + jj] = active ? E : 0.0;
+  __syncthreads();
+  affine_scan<THREADS, NL, false>(cM, cE, cIn, TPL);
+  __syncthreads();
+  // Forward fix-up, then the local backward recurrence x = rd y + n x'.
+  if (active) {
+    double y = cIn[ch * NL + jj];
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+      const int64_t i = s + k;
+      if (i < e) {
+        y = d[i * NL + jj] + mreg[k] * y;
+        d[i * NL + jj] = y;
+      }
+    }
+    M = 1.0;
+    E = 0.0;
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k) {
+      const int64_t i = s + k;
+      if (i < e) {
+        double a, b, cc;
+        li0_row(c, o, i, v, a, b, cc);
+        const double n = -cc * rdreg[k];
+        mreg[k] = n;
+        E = rdreg[k] * d[i * NL + jj] + n * E;
+        M = n * M;
+      }
+    }
+  }
+  __syncthreads();  // everyone done reading cIn of the forward scan
+  cM[ch * NL + jj] = active ? M : 1.0;
+  cE[ch * NL + jj] = active ? E : 0.0;
+  __syncthreads();
+  affine_scan<THREADS, NL, true>(cM, cE, cIn, TPL);
+  __syncthreads();
+  if (active) {
+    double x = cIn[ch * NL + jj];
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k) {
+      const int64_t i = s + k;
+      if (i < e) {
+        x = rdreg[k] * d[i * NL + jj] + mreg[k] * x;
+        d[i * NL + jj] = x;
+      }
+    }
+  }
+  __syncthreads();
+
+  // Phase 3: fused stage output.
+  double *ob = out + o.off;
+  for (int64_t q = tid; q < nS * NL; q += THREADS) {
+    const int64_t i = q / NL, jq = j0 + q % NL;
+    if (jq < nv) ob[i * nv + jq] = spot_out<MODE>(c, o, i, jq, d[q], U, Y);
+  }
+}
+
+// ---- variance sweep -------------------------------------------------------------
+// LDS: d[NL * ldv] (line ii, position j at ii*ldv + j) + carries.
+template <int MODE, int THREADS, int NL, int MAXL>
+__global__ void __launch_bounds__(THREADS)
+var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ W,
+                 double *__restrict__ out, int64_t ldv) {
+  extern __shared__ double lds[];
+  constexpr int TPL = THREADS / NL;
+  double *d = lds;
+  double *cM = lds + NL * ldv;
+  double *cE = cM + TPL * NL;
+  double *cIn = cE + TPL * NL;
+  const Tile t = tiles[blockIdx.x];
+  const HOpt o = c.opts[t.opt];
+  const int64_t nS = o.nS, nv = o.nv, i0 = t.start;
+  const int tid = threadIdx.x;
+  const double *w = W + o.off;
+
+  // Phase 1: right-hand side (contiguous rows: coalesced).
+  for (int64_t q = tid; q < NL * nv; q += THREADS) {
+    const int64_t ii = q / nv, jq = q % nv, i = i0 + ii;
+    double val = 0.0;
+    if (i < nS) {
+      if (MODE == VM_IMPL) {
+        auto xw = [&](int64_t a, int64_t b) { return w[a * nv + b]; };
+        val = w[i * nv + jq] + o.dt * a01_dot(c, o, i, jq, xw);
+      } else {
+        val = w[i * nv + jq];
+      }
+    }
+    d[ii * ldv + jq] = val;
+  }
+  __syncthreads();
+
+  // Phase 2: lines with S != 0 share the 1-D factorisation; S = 0 is diagonal.
+  const int ii = tid % NL, ch = tid / NL;
+  const int64_t i = i0 + ii;
+  const int L = (int)((nv + TPL - 1) / TPL);
+  const int64_t s = (int64_t)ch * L;
+  const int64_t e = s + L < nv ? s + L : nv;
+  const bool active = i < nS && s < e;
+  const bool diag = active && SP(c, o, SP_S, i) == 0.0;
+  double *dl = d + ii * ldv;
+  double M = 1.0, E = 0.0;
+  // Bottom fold of the RHS: d[nv-1] += f d[nv-2] (rows are only read here).
+  if (active && !diag && e == nv) dl[nv - 1] += o.fold_f * dl[nv - 2];
+  __syncthreads();
+  if (active && !diag) {
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+      const int64_t jq = s + k;
+      if (jq < e) {
+        const double m = VT(c, o, VT_M, jq);
+        E = dl[jq] + m * E;
+        M = m * M;
+      }
+    }
+  }
+  cM[ch * NL + ii] = (active && !diag) ? M : 1.0;
+  cE[ch * NL + ii] = (active && !diag) ? E : 0.0;
+  __syncthreads();
+  affine_scan<THREADS, NL, false>(cM, cE, cIn, TPL);
+  __syncthreads();
+  if (active && !diag) {
+    double y = cIn[ch * NL + ii];
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+      const int64_t jq = s + k;
+      if (jq < e) {
+        y = dl[jq] + VT(c, o, VT_M, jq) * y;
+        dl[jq] = y;
+      }
+    }
+    M = 1.0;
+    E = 0.0;
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k) {
+      const int64_t jq = s + k;
+      if (jq < e) {
+        const double n = VT(c, o, VT_N, jq);
+        E = VT(c, o, VT_RD, jq) * dl[jq] + n * E;
+        M = n * M;
+      }
+    }
+  }
+  __syncthreads();
+  cM[ch * NL + ii] = (active && !diag) ? M : 1.0;
+  cE[ch * NL + ii] = (active && !diag) ? E : 0.0;
+  __syncthreads();
+  affine_scan<THREADS, NL, true>(cM, cE, cIn, TPL);
+  __syncthreads();
+  if (active && !diag) {
+    double x = cIn[ch * NL + ii];
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k) {
+      const int64_t jq = s + k;
+      if (jq < e) {
+        x = VT(c, o, VT_RD, jq) * dl[jq] + VT(c, o, VT_N, jq) * x;
+        dl[jq] = x;
+      }
+    }
+  } else if (diag) {
+    for (int64_t jq = s; jq < e; ++jq) dl[jq] *= o.rdS0;
+  }
+  __syncthreads();
+
+  // Phase 3: write back.
+  double *ob = out + o.off;
+  for (int64_t q = tid; q < NL * nv; q += THREADS) {
+    const int64_t iq = q / nv, jq = q % nv;
+    if (i0 + iq < nS) ob[(i0 + iq) * nv + jq] = d[iq * ldv + jq];
+  }
+}
+
+// ---- set-up kernels ---------------------------------------------------------------
+struct SetupOpt {
+  int64_t nS, nv, so, vo;  // mesh / table offsets
+};
+
+// One thread per spot node of every option: the (0,) and (0,0) template rows
+// (with boundary rows) scaled by r*S and 1/2*S^2, and the cross weights.
+__global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *row_opt,
+                                   const double *__restrict__ spots, double *sp, int64_t sp_n) {
+#pragma clang fp contract(off)
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= sp_n) return;
+  const int k = (int)row_opt[g];
+  const HOpt o = opts[k];
+  const int64_t i = g - o.so, n = o.nS;
+  const double *m = spots + o.so;
+  double w1[3] = {0, 0, 0}, w2[3] = {0, 0, 0}, ws[3] = {0, 0, 0};
+  double R1 = 0.0, R2 = 0.0;
+  const bool dir = (i == 0) || (o.barrier && i == n - 1);
+  if (i > 0 && i < n - 1) {
+    const double hm = m[i] - m[i - 1], hp = m[i + 1] - m[i], x = hm + hp;
+    w1[2] = hm / (hp * x);
+    w1[1] = (-hm + hp) / (hm * hp);
+    w1[0] = -hp / (hm * x);
+    w2[2] = 2 / (hp * x);
+    w2[1] = -2 / (hm * hp);
+    w2[0] = 2 / (hm * x);
+    ws[0] = w1[0]; ws[1] = w1[1]; ws[2] = w1[2];
+  } else if (i == n - 1 && !o.barrier) {
+    const double d = m[n - 1] - m[n - 2];
+    R1 = 1.0;                 // dU/dS = 1 (Von Neumann)
+    w2[1] = -2 / (d * d);     // second derivative with known first derivative
+    w2[0] = 2 / (d * d);
+    R2 = 2 / d;
+  }
+  const double S = m[i];
+  const double c0 = o.r * S;
+  const double S2h = 0.5 * (S * S);
+  double *col = sp + g;
+  col[SP_PSUB * sp_n] = dir ? 0.0 : w1[0] * c0;
+  col[SP_PMID * sp_n] = dir ? 0.0 : w1[1] * c0 + (-0.5 * o.r);
+  col[SP_PSUP * sp_n] = dir ? 0.0 : w1[2] * c0;
+  col[SP_QSUB * sp_n] = dir ? 0.0 : w2[0] * S2h;
+  col[SP_QMID * sp_n] = dir ? 0.0 : w2[1] * S2h;
+  col[SP_QSUP * sp_n] = dir ? 0.0 : w2[2] * S2h;
+  col[SP_PR * sp_n] = dir ? 0.0 : R1 * c0;
+  col[SP_QR * sp_n] = dir ? 0.0 : R2 * S2h;
+  col[SP_WSUB * sp_n] = ws[0];
+  col[SP_WMID * sp_n] = ws[1];
+  col[SP_WSUP * sp_n] = ws[2];
+  col[SP_S * sp_n] = S;
+  col[SP_DIR * sp_n] = dir ? 1.0 : 0.0;
+}
+
+// One thread per variance node: A1 rows for S != 0 lines and cross weights.
+__global__ void var_tables_kernel(const HOpt *opts, const int64_t *row_opt,
+                                  const double *__restrict__ vars, double *vt, int64_t vt_n) {
+#pragma clang fp contract(off)
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= vt_n) return;
+  const int k = (int)row_opt[g];
+  const HOpt o = opts[k];
+  const int64_t j = g - o.vo, n = o.nv;
+  const double *m = vars + o.vo;
+  double u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0}, wv[3] = {0, 0, 0};
+  if (j > 0 && j < n - 1) {
+    const double hm = m[j] - m[j - 1], hp = m[j + 1] - m[j], x = hm + hp;
+    u1[2] = hm / (hp * x);
+    u1[1] = (-hm + hp) / (hm * hp);
+    u1[0] = -hp / (hm * x);
+    u2[2] = 2 / (hp * x);
+    u2[1] = -2 / (hm * hp);
+    u2[0] = 2 / (hm * x);
+    wv[0] = u1[0]; wv[1] = u1[1]; wv[2] = u1[2];
+  } else if (j == 0) {
+    const double d1 = m[1] - m[0];
+    u1[2] = 1 / d1;   // one-sided first derivative; d2/dv2 row is Von Neumann 0
+    u1[1] = -1 / d1;
+  } else {
+    const double d1 = m[n - 1] - m[n - 2], d2 = m[n - 2] - m[n - 3];
+    u1[1] = 1.0 / d1;
+    u1[0] = -1.0 / d1;
+    const double rd = 1.0 / (0.5 * (d2 + d1) * d2 * d1);
+    u2[0] = -(d2 + d1) * rd;
+    u2[1] = d2 * rd;
+  }
+  const double v = m[j];
+  const double c1 = o.kappa * (o.theta - v);
+  const double c11 = (0.5 * (o.sigma * o.sigma)) * v;
+  double *col = vt + g;
+  col[VT_GSUB * vt_n] = u1[0] * c1 + u2[0] * c11;
+  col[VT_GMID * vt_n] = u1[1] * c1 + u2[1] * c11;
+  col[VT_GSUP * vt_n] = u1[2] * c1 + u2[2] * c11;
+  col[VT_WSUB * vt_n] = wv[0];
+  col[VT_WMID * vt_n] = wv[1];
+  col[VT_WSUP * vt_n] = wv[2];
+  col[VT_V * vt_n] = v;
+}
+
+// Per option (one thread): Gz, the folded LU of I - tdt*A1 for S != 0 lines.
+__global__ void var_factor_kernel(HOpt *opts, int nopt, const double *__restrict__ vars,
+                                  double *vt, int64_t vt_n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nopt) return;
+  HOpt o = opts[k];
+  const int64_t n = o.nv;
+  const double *m = vars + o.vo;
+  const double *G = vt;
+  auto g = [&](int f, int64_t j) { return G[f * vt_n + o.vo + j]; };
+  {
+    const double d1 = m[n - 1] - m[n - 2], d2 = m[n - 2] - m[n - 3];
+    const double rd = 1.0 / (0.5 * (d2 + d1) * d2 * d1);
+    o.Gz = (d1 * rd) * ((0.5 * (o.sigma * o.sigma)) * m[n - 1]);
+  }
+  const double t = o.tdt;
+  auto row = [&](int64_t j, double &a, double &b, double &c) {
+    a = j > 0 ? -t * g(VT_GSUB, j) : 0.0;
+    b = 1.0 - t * (g(VT_GMID, j) - 0.5 * o.r);
+    c = j < n - 1 ? -t * g(VT_GSUP, j) : 0.0;
+  };
+  // Fold of the last row: f = -zL / aL[n-2].
+  double a2, b2, c2;
+  row(n - 2, a2, b2, c2);
+  const double zL = -t * o.Gz;
+  const double f = a2 == 0.0 ? 0.0 : -zL / a2;
+  o.fold_f = f;
+  o.rdS0 = 1.0 / (1.0 - t * (-0.5 * o.r));
+  double *M = vt + VT_M * vt_n + o.vo, *RD = vt + VT_RD * vt_n + o.vo,
+         *N = vt + VT_N * vt_n + o.vo;
+  double rdprev = 0.0, cprev = 0.0;
+  for (int64_t j = 0; j < n; ++j) {
+    double a, b, c;
+    row(j, a, b, c);
+    if (j == n - 1) {
+      a += f * b2;
+      b += f * c2;
+    }
+    const double l = a * rdprev;
+    const double rdj = 1.0 / (b - l * cprev);
+    M[j] = -l;
+    RD[j] = rdj;
+    N[j] = -c * rdj;
+    rdprev = rdj;
+    cprev = c;
+  }
+  opts[k] = o;
+}
+
+// One thread per spot line (option, column): pivots of I - tdt*A0 down the line.
+__global__ void spot_factor_kernel(Ctx c, const Tile *__restrict__ tiles, int ntiles,
+                                   double *rd0) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t tix = g / 64;
+  if (tix >= ntiles) return;
+  const Tile t = tiles[tix];
+  const HOpt o = c.opts[t.opt];
+  const int64_t j = t.start + (g % 64);
+  if (j >= o.nv) return;
+  const double v = c.vt[VT_V * c.vt_n + o.vo + j];
+  double *rd = rd0 + o.off;
+  double rdprev = 0.0, cprev = 0.0;
+  for (int64_t i = 0; i < o.nS; ++i) {
+    double a, b, cc;
+    li0_row(c, o, i, v, a, b, cc);
+    const double rdi = 1.0 / (b - (a * rdprev) * cprev);
+    rd[i * o.nv + j] = rdi;
+    rdprev = rdi;
+    cprev = cc;
+  }
+}
+
+}  // namespace
+}  // namespace fdadi
+
+using namespace fdadi;
+
+struct fdadi_hb {
+  int nopt = 0;
+  std::vector<HOpt> h_opts;  // host copy (dt of each option before dt_scale)
+  std::vector<double> base_dt;
+  HOpt *d_opts = nullptr;
+  double *d_spots = nullptr, *d_vars = nullptr;
+  double *d_sp = nullptr, *d_vt = nullptr;
+  int64_t sp_n = 0, vt_n = 0, total = 0;
+  int64_t nSmax = 0, nvmax = 0;
+  double *d_state = nullptr, *d_w1 = nullptr, *d_y2 = nullptr, *d_rd0 = nullptr;
+  int64_t *d_sp_row = nullptr, *d_vt_row = nullptr;
+  Tile *d_stiles = nullptr, *d_vtiles = nullptr, *d_ftiles = nullptr;
+  int nstiles = 0, nvtiles = 0, nftiles = 0;
+  int spot_threads = 0, var_threads = 0;
+  int scheme = -1;
+  double theta = 0.5, dt_scale = 1.0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace fdadi {
+namespace {
+
+constexpr int kNL = 8;
+constexpr int kMAXL = 16;
+
+Ctx ctx_of(const fdadi_hb *hb) {
+  Ctx c;
+  c.opts = hb->d_opts;
+  c.sp = hb->d_sp;
+  c.vt = hb->d_vt;
+  c.sp_n = hb->sp_n;
+  c.vt_n = hb->vt_n;
+  c.rd0 = hb->d_rd0;
+  return c;
+}
+
+#define FDADI_RET(x)         \
+  do {                       \
+    int _st = (x);           \
+    if (_st) return _st;     \
+  } while (0)
+
+// Kernels using more than 64 KB of dynamic LDS must say so once.
+int allow_lds(const void *fn, size_t bytes) {
+  if (bytes <= 65536) return FDADI_OK;
+  FDADI_REQUIRE(bytes <= 160 * 1024, FDADI_ERR_UNSUPPORTED,
+                "sweep tile needs " + std::to_string(bytes) + " bytes of LDS (> 160 KB)");
+  FDADI_TRY_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return FDADI_OK;
+}
+
+int threads_for(int64_t n) {
+  // Threads per workgroup so that each line's chunk fits MAXL registers.
+  const int64_t need = (n + kMAXL - 1) / kMAXL * kNL;
+  int t = 64;
+  while (t < need && t < 1024) t *= 2;
+  return t;
+}
+
+template <int MODE, int T>
+int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *out,
+                  hipStream_t s) {
+  constexpr int TPL = T / kNL;
+  const size_t lds = sizeof(double) * (hb->nSmax * kNL + 3 * TPL * kNL);
+  FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, kNL, kMAXL>, lds));
+  hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, kNL, kMAXL>), dim3(hb->nstiles), dim3(T), lds,
+                     s, ctx_of(hb), hb->d_stiles, U, Y, out, hb->nSmax);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+template <int MODE>
+int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *out,
+                hipStream_t s) {
+  switch (hb->spot_threads) {
+    case 64: return launch_spot_t<MODE, 64>(hb, U, Y, out, s);
+    case 128: return launch_spot_t<MODE, 128>(hb, U, Y, out, s);
+    case 256: return launch_spot_t<MODE, 256>(hb, U, Y, out, s);
+    case 512: return launch_spot_t<MODE, 512>(hb, U, Y, out, s);
+    case 1024: return launch_spot_t<MODE, 1024>(hb, U, Y, out, s);
+  }
+  set_error("spot sweep: unsupported thread count");
+  return FDADI_ERR_UNSUPPORTED;
+}
+
+template <int MODE, int T>
+int launch_var_t(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
+  constexpr int TPL = T / kNL;
+  const int64_t ldv = hb->nvmax + 1;
+  const size_t lds = sizeof(double) * (kNL * ldv + 3 * TPL * kNL);
+  FDADI_RET(allow_lds((const void *)var_sweep_kernel<MODE, T, kNL, kMAXL>, lds));
+  hipLaunchKernelGGL((var_sweep_kernel<MODE, T, kNL, kMAXL>), dim3(hb->nvtiles), dim3(T), lds, s,
+                     ctx_of(hb), hb->d_vtiles, W, out, ldv);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+template <int MODE>
+int launch_var(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
+  switch (hb->var_threads) {
+    case 64: return launch_var_t<MODE, 64>(hb, W, out, s);
+    case 128: return launch_var_t<MODE, 128>(hb, W, out, s);
+    case 256: return launch_var_t<MODE, 256>(hb, W, out, s);
+    case 512: return launch_var_t<MODE, 512>(hb, W, out, s);
+    case 1024: return launch_var_t<MODE, 1024>(hb, W, out, s);
+  }
+  set_error("variance sweep: unsupported thread count");
+  return FDADI_ERR_UNSUPPORTED;
+}
+
+// One time step of the configured scheme (stage list of the module comment).
+int step_once(fdadi_hb *hb, hipStream_t s) {
+  double *U = hb->d_state, *W = hb->d_w1, *Y2 = hb->d_y2;
+  switch (hb->scheme) {
+    case FDADI_SCHEME_DOUGLAS:
+      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
+      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      return FDADI_OK;
+    case FDADI_SCHEME_IMPLICIT:
+      FDADI_RET(launch_var<VM_IMPL>(hb, U, W, s));
+      FDADI_RET(launch_spot<SM_IMPL>(hb, W, nullptr, U, s));
+      return FDADI_OK;
+    case FDADI_SCHEME_HV:
+      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
+      FDADI_RET(launch_var<VM_PLAIN>(hb, W, Y2, s));
+      FDADI_RET(launch_spot<SM_HV>(hb, U, Y2, W, s));
+      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      return FDADI_OK;
+    case FDADI_SCHEME_CS:
+      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
+      FDADI_RET(launch_var<VM_PLAIN>(hb, W, Y2, s));
+      FDADI_RET(launch_spot<SM_CS>(hb, U, Y2, W, s));
+      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      return FDADI_OK;
+    case FDADI_SCHEME_MCS:
+      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
+      FDADI_RET(launch_var<VM_PLAIN>(hb, W, Y2, s));
+      FDADI_RET(launch_spot<SM_MCS>(hb, U, Y2, W, s));
+      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      return FDADI_OK;
+  }
+  set_error("fdadi_hb_step: engine not configured (call fdadi_hb_configure)");
+  return FDADI_ERR_STATE;
+}
+
+void free_hb(fdadi_hb *hb) {
+  if (!hb) return;
+  if (hb->gexec) (void)hipGraphExecDestroy(hb->gexec);
+  if (hb->graph) (void)hipGraphDestroy(hb->graph);
+  void *ptrs[] = {hb->d_opts, hb->d_spots, hb->d_vars, hb->d_sp, hb->d_vt, hb->d_state,
+                  hb->d_w1, hb->d_y2, hb->d_rd0, hb->d_sp_row, hb->d_vt_row, hb->d_stiles,
+                  hb->d_vtiles, hb->d_ftiles};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete hb;
+}
+
+}  // namespace
+}  // namespace fdadi
+
+extern "C" {
+
+int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double *spots,
+                    const double *vars, void *stream, fdadi_hb **out) {
+  FDADI_REQUIRE(out && opts && spots && vars && nopt > 0, FDADI_ERR_ARG,
+                "fdadi_hb_create: bad arguments");
+  hipStream_t s = as_stream(stream);
+  fdadi_hb *hb = new fdadi_hb();
+  hb->nopt = nopt;
+  hb->h_opts.resize(nopt);
+  hb->base_dt.resize(nopt);
+  int64_t so = 0, vo = 0, off = 0;
+  std::vector<int64_t> sp_row, vt_row;
+  for (int k = 0; k < nopt; ++k) {
+    const fdadi_heston_option &q = opts[k];
+    if (q.nS < 4 || q.nv < 4) {
+      free_hb(hb);
+      set_error("fdadi_hb_create: option " + std::to_string(k) +
+                ": grid needs at least 4 points per axis");
+      return FDADI_ERR_ARG;
+    }
+    if (q.nS > 2048 || q.nv > 2048) {
+      free_hb(hb);
+      set_error("fdadi_hb_create: option " + std::to_string(k) +
+                ": the fused engine supports nS, nv <= 2048");
+      return FDADI_ERR_UNSUPPORTED;
+    }
+    HOpt &o = hb->h_opts[k];
+    o = HOpt{};
+    o.nS = q.nS;
+    o.nv = q.nv;
+    o.off = off;
+    o.so = so;
+    o.vo = vo;
+    o.r = q.r;
+    o.kappa = q.kappa;
+    o.theta = q.theta;
+    o.sigma = q.sigma;
+    o.rho = q.rho;
+    o.dt = q.dt;
+    o.barrier = q.barrier ? 1 : 0;
+    hb->base_dt[k] = q.dt;
+    for (int64_t i = 0; i < q.nS; ++i) sp_row.push_back(k);
+    for (int64_t j = 0; j < q.nv; ++j) vt_row.push_back(k);
+    so += q.nS;
+    vo += q.nv;
+    off += q.nS * q.nv;
+    hb->nSmax = std::max<int64_t>(hb->nSmax, q.nS);
+    hb->nvmax = std::max<int64_t>(hb->nvmax, q.nv);
+  }
+  hb->sp_n = so;
+  hb->vt_n = vo;
+  hb->total = off;
+  // Tiles, largest options first (the dispatcher then balances the tail).
+  std::vector<int> order(nopt);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return hb->h_opts[a].nS * hb->h_opts[a].nv > hb->h_opts[b].nS * hb->h_opts[b].nv;
+  });
+  std::vector<Tile> st, vtl, ft;
+  for (int k : order) {
+    const HOpt &o = hb->h_opts[k];
+    for (int64_t j0 = 0; j0 < o.nv; j0 += kNL) st.push_back(Tile{k, (int32_t)j0});
+    for (int64_t i0 = 0; i0 < o.nS; i0 += kNL) vtl.push_back(Tile{k, (int32_t)i0});
+    for (int64_t j0 = 0; j0 < o.nv; j0 += 64) ft.push_back(Tile{k, (int32_t)j0});
+  }
+  hb->nstiles = (int)st.size();
+  hb->nvtiles = (int)vtl.size();
+  hb->nftiles = (int)ft.size();
+  hb->spot_threads = threads_for(hb->nSmax);
+  hb->var_threads = threads_for(hb->nvmax);
+  hipError_t e = hipSuccess;
+  auto dalloc = [&](void **p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 8);
+  };
+  dalloc((void **)&hb->d_opts, sizeof(HOpt) * nopt);
+  dalloc((void **)&hb->d_spots, sizeof(double) * so);
+  dalloc((void **)&hb->d_vars, sizeof(double) * vo);
+  dalloc((void **)&hb->d_sp, sizeof(double) * SP_NF * so);
+  dalloc((void **)&hb->d_vt, sizeof(double) * VT_NF * vo);
+  dalloc((void **)&hb->d_state, sizeof(double) * off);
+  dalloc((void **)&hb->d_w1, sizeof(double) * off);
+  dalloc((void **)&hb->d_y2, sizeof(double) * off);
+  dalloc((void **)&hb->d_rd0, sizeof(double) * off);
+  dalloc((void **)&hb->d_sp_row, sizeof(int64_t) * so);
+  dalloc((void **)&hb->d_vt_row, sizeof(int64_t) * vo);
+  dalloc((void **)&hb->d_stiles, sizeof(Tile) * st.size());
+  dalloc((void **)&hb->d_vtiles, sizeof(Tile) * vtl.size());
+  dalloc((void **)&hb->d_ftiles, sizeof(Tile) * ft.size());
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  up(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * nopt);
+  up(hb->d_spots, spots, sizeof(double) * so);
+  up(hb->d_vars, vars, sizeof(double) * vo);
+  up(hb->d_sp_row, sp_row.data(), sizeof(int64_t) * so);
+  up(hb->d_vt_row, vt_row.data(), sizeof(int64_t) * vo);
+  up(hb->d_stiles, st.data(), sizeof(Tile) * st.size());
+  up(hb->d_vtiles, vtl.data(), sizeof(Tile) * vtl.size());
+  up(hb->d_ftiles, ft.data(), sizeof(Tile) * ft.size());
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(spot_tables_kernel, dim3(grid_for(so, 256)), dim3(256), 0, s,
+                       hb->d_opts, nopt, hb->d_sp_row, hb->d_spots, hb->d_sp, so);
+    hipLaunchKernelGGL(var_tables_kernel, dim3(grid_for(vo, 256)), dim3(256), 0, s, hb->d_opts,
+                       hb->d_vt_row, hb->d_vars, hb->d_vt, vo);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    free_hb(hb);
+    set_error(std::string("fdadi_hb_create: ") + hipGetErrorString(e));
+    return FDADI_ERR_HIP;
+  }
+  *out = hb;
+  return FDADI_OK;
+}
+
+int fdadi_hb_destroy(fdadi_hb *hb) {
+  free_hb(hb);
+  return FDADI_OK;
+}
+
+int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale, void *stream) {
+  FDADI_REQUIRE(hb, FDADI_ERR_ARG, "fdadi_hb_configure: null engine");
+  FDADI_REQUIRE(scheme >= FDADI_SCHEME_IMPLICIT && scheme <= FDADI_SCHEME_HV, FDADI_ERR_ARG,
+                "fdadi_hb_configure: unknown scheme");
+  FDADI_REQUIRE(dt_scale > 0, FDADI_ERR_ARG, "fdadi_hb_configure: dt_scale must be positive");
+  hipStream_t s = as_stream(stream);
+  hb->scheme = scheme;
+  hb->theta = theta;
+  hb->dt_scale = dt_scale;
+  for (int k = 0; k < hb->nopt; ++k) {
+    HOpt &o = hb->h_opts[k];
+    o.dt = hb->base_dt[k] * dt_scale;
+    o.tdt = scheme == FDADI_SCHEME_IMPLICIT ? o.dt : theta * o.dt;
+  }
+  FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
+                               hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(var_factor_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0, s, hb->d_opts,
+                     hb->nopt, hb->d_vars, hb->d_vt, hb->vt_n);
+  FDADI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(spot_factor_kernel, dim3(hb->nftiles), dim3(64), 0, s, ctx_of(hb),
+                     hb->d_ftiles, hb->nftiles, hb->d_rd0);
+  FDADI_CHECK_LAUNCH();
+  // Keep the host copy in step with the factor constants written on the device.
+  FDADI_TRY_HIP(hipMemcpyAsync(hb->h_opts.data(), hb->d_opts, sizeof(HOpt) * hb->nopt,
+                               hipMemcpyDeviceToHost, s));
+  FDADI_TRY_HIP(hipStreamSynchronize(s));
+  if (hb->gexec) {
+    (void)hipGraphExecDestroy(hb->gexec);
+    hb->gexec = nullptr;
+  }
+  if (hb->graph) {
+    (void)hipGraphDestroy(hb->graph);
+    hb->graph = nullptr;
+  }
+  return FDADI_OK;
+}
+
+int fdadi_hb_state(fdadi_hb *hb, double **d_state, int64_t *total) {
+  FDADI_REQUIRE(hb, FDADI_ERR_ARG, "fdadi_hb_state: null engine");
+  if (d_state) *d_state = hb->d_state;
+  if (total) *total = hb->total;
+  return FDADI_OK;
+}
+
+int fdadi_hb_offsets(const fdadi_hb *hb, int64_t *offsets) {
+  FDADI_REQUIRE(hb && offsets, FDADI_ERR_ARG, "fdadi_hb_offsets: bad arguments");
+  for (int k = 0; k < hb->nopt; ++k) offsets[k] = hb->h_opts[k].off;
+  return FDADI_OK;
+}
+
+int fdadi_hb_step(fdadi_hb *hb, int32_t nsteps, void *stream) {
+  FDADI_REQUIRE(hb, FDADI_ERR_ARG, "fdadi_hb_step: null engine");
+  FDADI_REQUIRE(nsteps >= 0, FDADI_ERR_ARG, "fdadi_hb_step: negative step count");
+  hipStream_t s = as_stream(stream);
+  for (int32_t n = 0; n < nsteps; ++n) FDADI_RET(step_once(hb, s));
+  return FDADI_OK;
+}
+
+int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream) {
+  FDADI_REQUIRE(hb, FDADI_ERR_ARG, "fdadi_hb_capture: null engine");
+  FDADI_REQUIRE(nsteps > 0, FDADI_ERR_ARG, "fdadi_hb_capture: step count must be positive");
+  FDADI_REQUIRE(hb->scheme >= 0, FDADI_ERR_STATE, "fdadi_hb_capture: engine not configured");
+  FDADI_REQUIRE(stream, FDADI_ERR_ARG, "fdadi_hb_capture: needs a created stream");
+  hipStream_t s = as_stream(stream);
+  if (hb->gexec) {
+    (void)hipGraphExecDestroy(hb->gexec);
+    hb->gexec = nullptr;
+  }
+  if (hb->graph) {
+    (void)hipGraphDestroy(hb->graph);
+    hb->graph = nullptr;
+  }
+  FDADI_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int st = FDADI_OK;
+  for (int32_t n = 0; n < nsteps && st == FDADI_OK; ++n) st = step_once(hb, s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (st != FDADI_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("fdadi_hb_capture: ") + hipGetErrorString(e));
+    return FDADI_ERR_HIP;
+  }
+  hb->graph = g;
+  FDADI_TRY_HIP(hipGraphInstantiate(&hb->gexec, g, nullptr, nullptr, 0));
+  return FDADI_OK;
+}
+
+int fdadi_hb_replay(fdadi_hb *hb, void *stream) {
+  FDADI_REQUIRE(hb && hb->gexec, FDADI_ERR_STATE, "fdadi_hb_replay: no captured graph");
+  FDADI_TRY_HIP(hipGraphLaunch(hb->gexec, as_stream(stream)));
+  return FDADI_OK;
+}
+
+int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
+  FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_set_shape: bad kind");
+  FDADI_REQUIRE(lines == 0 || lines == kNL, FDADI_ERR_UNSUPPORTED,
+                "fdadi_hb_set_shape: lines per tile is fixed at 8");
+  const int64_t n = kind == 0 ? hb->nSmax : hb->nvmax;
+  int t = threads == 0 ? threads_for(n) : threads;
+  FDADI_REQUIRE(t == 64 || t == 128 || t == 256 || t == 512 || t == 1024, FDADI_ERR_ARG,
+                "fdadi_hb_set_shape: threads must be a power of two in [64, 1024]");
+  FDADI_REQUIRE((int64_t)(t / kNL) * kMAXL >= n, FDADI_ERR_ARG,
+                "fdadi_hb_set_shape: too few threads for the longest line");
+  if (kind == 0) hb->spot_threads = t; else hb->var_threads = t;
+  if (hb->gexec) {
+    (void)hipGraphExecDestroy(hb->gexec);
+    hb->gexec = nullptr;
+  }
+  if (hb->graph) {
+    (void)hipGraphDestroy(hb->graph);
+    hb->graph = nullptr;
+  }
+  return FDADI_OK;
+}
+
+int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines) {
+  FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_get_shape: bad kind");
+  if (threads) *threads = kind == 0 ? hb->spot_threads : hb->var_threads;
+  if (lines) *lines = kNL;
+  return FDADI_OK;
+}
+
+int fdadi_hb_launches_per_step(const fdadi_hb *hb, int *n) {
+  FDADI_REQUIRE(hb && n, FDADI_ERR_ARG, "fdadi_hb_launches_per_step: bad arguments");
+  switch (hb->scheme) {
+    case FDADI_SCHEME_DOUGLAS:
+    case FDADI_SCHEME_IMPLICIT: *n = 2; break;
+    default: *n = 4; break;
+  }
+  return FDADI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+/*
+ * fdadi.h -- C ABI of the MI355X-native ADI finite-difference engine.
+ *
+ * Every function returns FDADI_OK (0) or a nonzero status; the message of the
+ * last failure on the calling thread is returned by fdadi_last_error().
+ * Pointers named d_* / *_dev are device pointers; all others are host
+ * pointers.  `stream` is a hipStream_t passed as void* (NULL = the library's
+ * default stream, which is the null stream).  Launches are asynchronous;
+ * only the *_d2h / *_read functions synchronise.  All arithmetic is FP64.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference checkout, FiniteDifference/ omitted).
+ */
+#ifndef FDADI_H
+#define FDADI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FDADI_OK = 0,
+  FDADI_ERR_ARG = 1,          /* bad argument / size mismatch (reference: DIE -> domain_error) */
+  FDADI_ERR_STATE = 2,        /* operation not allowed in this state (folded, mixed, ...) */
+  FDADI_ERR_HIP = 3,          /* a HIP runtime call failed */
+  FDADI_ERR_UNSUPPORTED = 4   /* configuration not supported */
+};
+
+/* Fold states: common.h:34-36 FOLDED / CAN_FOLD / CANNOT_FOLD strings. */
+enum { FDADI_CANNOT_FOLD = 0, FDADI_CAN_FOLD = 1, FDADI_FOLDED = 2 };
+
+/* Schemes of the fused engine (FiniteDifferenceEngine.pyx:616-1016). */
+enum {
+  FDADI_SCHEME_IMPLICIT = 0,  /* solve_implicit            :616-659  */
+  FDADI_SCHEME_DOUGLAS = 1,   /* solve_douglas             :944-1006 */
+  FDADI_SCHEME_CS = 2,        /* solve_craigsneyd          :842-898  */
+  FDADI_SCHEME_MCS = 3,       /* solve_craigsneyd2         :778-839  */
+  FDADI_SCHEME_HV = 4         /* solve_hundsdorferverwer   :699-775  */
+};
+
+/* ---- runtime ------------------------------------------------------------- */
+/* Replaces the DIE()/std::domain_error convention, common.h:17-27. */
+const char *fdadi_last_error(void);
+int fdadi_version(int *major, int *minor);
+int fdadi_device_count(int *n);
+int fdadi_set_device(int dev);
+int fdadi_stream_create(void **stream);
+int fdadi_stream_destroy(void *stream);
+int fdadi_stream_sync(void *stream);
+int fdadi_device_sync(void);
+
+/* ---- device arrays: SizedArray<double>, VecArray.h:211-710 ----------------- */
+/* SizedArray ctors / device_malloc, VecArray.h:232-361. */
+int fdadi_array_alloc(int64_t n, double **d_out);
+int fdadi_array_free(double *d_ptr);
+/* to_SizedArray H2D copy, SizedArrayPtr.pyx:320-326. */
+int fdadi_array_h2d(double *d_dst, const double *src, int64_t n, void *stream);
+/* from_SizedArray / to_numpy D2H copy, SizedArrayPtr.pyx:349-357 (synchronous). */
+int fdadi_array_d2h(double *dst, const double *d_src, int64_t n, void *stream);
+/* copy_from, VecArray.h:400-410. */
+int fdadi_array_copy(double *d_dst, const double *d_src, int64_t n, void *stream);
+int fdadi_array_fill(double *d_dst, double value, int64_t n, void *stream);
+/* Elementwise a op= b: op 0 pluseq, 1 minuseq, 2 minuseq_over2 (a=(a-b)/2),
+ * 3 timeseq (VecArray.cu:29-196). */
+int fdadi_array_binop(double *d_a, const double *d_b, int64_t n, int op, void *stream);
+/* Scalar variants (pluseq_scalar etc.), host value / device value
+ * (VecArray.h:632-689 *_from_host and device-scalar forms). */
+int fdadi_array_scalar_op(double *d_a, double s, int64_t n, int op, void *stream);
+int fdadi_array_scalar_op_dev(double *d_a, const double *d_s, int64_t n, int op, void *stream);
+/* transpose (h,w)->(w,h), VecArray.h:438-464 / _kernels.cu:7-37. */
+int fdadi_array_transpose(double *d_dst, const double *d_src, int64_t h, int64_t w, void *stream);
+
+/* ---- arena: one device allocation carved into many buffers --------------- */
+/* Replaces the per-SizedArray device_malloc + tempspace pair (VecArray.h:237). */
+typedef struct fdadi_arena fdadi_arena;
+int fdadi_arena_create(int64_t bytes, fdadi_arena **out);
+int fdadi_arena_alloc(fdadi_arena *a, int64_t n_doubles, double **d_out);
+int fdadi_arena_used(const fdadi_arena *a, int64_t *bytes);
+int fdadi_arena_reset(fdadi_arena *a);
+int fdadi_arena_destroy(fdadi_arena *a);
+
+/* ---- tridiagonal operator: _TriBandedOperator, _TriBandedOperatorGPU.cuh:17-87 */
+typedef struct fdadi_tri fdadi_tri;
+/* Constructor from host data, _TriBandedOperatorGPU.cuh:57-73 (emigrate_tri,
+ * BandedOperatorGPU.pyx:264-314).  Bands are row-indexed (sup[i]=A[i,i+1],
+ * sub[i]=A[i,i-1]), `rows` entries each; dirichlet/factor arrays have `blocks`
+ * entries; deltas has ndeltas entries (may be 0). */
+int fdadi_tri_create(const double *sup, const double *mid, const double *sub,
+                     const double *R, const double *deltas, int64_t ndeltas,
+                     const double *low_dirichlet, const double *high_dirichlet,
+                     const double *top_factors, const double *bottom_factors,
+                     int axis, int64_t rows, int64_t blocks, int has_low_dirichlet,
+                     int has_high_dirichlet, int top_fold_status,
+                     int bottom_fold_status, int has_residual, fdadi_tri **out);
+/* for_vector, _TriBandedOperatorGPU.cu:1107-1182: Heston 1-D central stencil
+ * with its boundary rows, built on the device from the host mesh. */
+int fdadi_tri_for_vector(const double *mesh, int64_t n, int64_t blocks, int derivative,
+                         int axis, int barrier, void *stream, fdadi_tri **out);
+int fdadi_tri_copy(const fdadi_tri *src, void *stream, fdadi_tri **out);
+int fdadi_tri_destroy(fdadi_tri *op);
+int fdadi_tri_info(const fdadi_tri *op, int64_t *rows, int64_t *blocks, int *axis,
+                   int *has_low_dirichlet, int *has_high_dirichlet, int *top_fold_status,
+                   int *bottom_fold_status, int *has_residual, int64_t *ndeltas);
+/* Public fields set from Python (has_low/high_dirichlet setters
+ * BandedOperatorGPU.pyx:45-68, enable_residual :463-467). */
+int fdadi_tri_set_flags(fdadi_tri *op, int has_low_dirichlet, int has_high_dirichlet,
+                        int has_residual);
+/* Public fields read back by immigrate_tri, BandedOperatorGPU.pyx:317-372.
+ * Any output pointer may be NULL.  Synchronous. */
+int fdadi_tri_read(const fdadi_tri *op, double *sup, double *mid, double *sub, double *R,
+                   double *low_dirichlet, double *high_dirichlet, double *top_factors,
+                   double *bottom_factors, double *deltas);
+/* apply, _TriBandedOperatorGPU.cu:463-553: d_y = A x (+R) with the Dirichlet
+ * values forced into x; the state is (nS,nv) row-major and the operator's axis
+ * is addressed with strides (no transposes).  d_x is not modified; d_y must not
+ * alias d_x.  A CAN_FOLD (unfolded) operator is applied with its +-2 entries. */
+int fdadi_tri_apply(const fdadi_tri *op, const double *d_x, double *d_y, int64_t n,
+                    void *stream);
+/* solve, _TriBandedOperatorGPU.cu:719-770: d_v <- A^-1 (v - R) per grid line,
+ * Dirichlet rows forced, RHS folded as needed (replaces
+ * cusparseDgtsvStridedBatch and the transposes). */
+int fdadi_tri_solve(const fdadi_tri *op, double *d_v, int64_t n, void *stream);
+/* add_operator, _TriBandedOperatorGPU.cu:577-671. */
+int fdadi_tri_add_operator(fdadi_tri *op, const fdadi_tri *other, void *stream);
+/* add_scalar_from_host :674-689 and add_scalar (device value) :696-711. */
+int fdadi_tri_add_scalar(fdadi_tri *op, double value, void *stream);
+int fdadi_tri_add_scalar_dev(fdadi_tri *op, const double *d_value, void *stream);
+/* vectorized_scale, _TriBandedOperatorGPU.cu:1012-1105: rows scaled by the
+ * device vector tiled with period vsize (vsize divides rows; 1 = scalar). */
+int fdadi_tri_scale(fdadi_tri *op, const double *d_vec, int64_t vsize, void *stream);
+int fdadi_tri_scale_scalar(fdadi_tri *op, double value, void *stream);
+/* fold_top / fold_bottom, _TriBandedOperatorGPU.cu:895-1009. */
+int fdadi_tri_fold_top(fdadi_tri *op, int unfold, void *stream);
+int fdadi_tri_fold_bottom(fdadi_tri *op, int unfold, void *stream);
+/* diagonalize / undiagonalize, _TriBandedOperatorGPU.cu:836-867. */
+int fdadi_tri_diagonalize(fdadi_tri *op, void *stream);
+int fdadi_tri_undiagonalize(fdadi_tri *op, void *stream);
+
+/* ---- cross-derivative operator: _CSRBandedOperator, _CSRBandedOperatorGPU.cuh:17-84 */
+typedef struct fdadi_cross fdadi_cross;
+/* mixed_for_vector, _CSRBandedOperatorGPU.cu:505-774: 9-point d2/dS dv. */
+int fdadi_cross_for_vector(const double *mesh0, int64_t n0, const double *mesh1, int64_t n1,
+                           void *stream, fdadi_cross **out);
+/* Constructor from host data (emigrate_csr, BandedOperatorGPU.pyx:176-232):
+ * bands[k*N + r] = A[r, r + off_k], off_k = a*n1 + b for a,b in (1,0,-1). */
+int fdadi_cross_create(const double *bands, const double *R, int64_t n0, int64_t n1,
+                       fdadi_cross **out);
+int fdadi_cross_copy(const fdadi_cross *src, void *stream, fdadi_cross **out);
+int fdadi_cross_destroy(fdadi_cross *op);
+int fdadi_cross_info(const fdadi_cross *op, int64_t *n0, int64_t *n1);
+int fdadi_cross_read(const fdadi_cross *op, double *bands, double *R);
+/* apply, _CSRBandedOperatorGPU.cu:340-426 (replaces cusparseDcsrmv). */
+int fdadi_cross_apply(const fdadi_cross *op, const double *d_x, double *d_y, int64_t n,
+                      void *stream);
+/* vectorized_scale, _CSRBandedOperatorGPU.cu:442-480. */
+int fdadi_cross_scale(fdadi_cross *op, const double *d_vec, int64_t vsize, void *stream);
+int fdadi_cross_scale_scalar(fdadi_cross *op, double value, void *stream);
+
+/* ---- Heston coefficient fields: _coefficients.cu:122-338 --------------------
+ * which: 0 = (0,) r*S, 1 = (0,0) 0.5*v*S^2, 2 = (1,) kappa*(theta-v)*[S!=0],
+ * 3 = (1,1) 0.5*sigma^2*v*[S!=0], 4 = (0,1) rho*sigma*S*v.  Output ordered
+ * with the derivative's axis fastest (spot-fastest for 0/1, var-fastest else).
+ * p = {r, kappa, theta, sigma, rho}. */
+int fdadi_heston_coefficients(int which, const double *d_spots, int64_t nS,
+                              const double *d_vars, int64_t nv, const double *p,
+                              double *d_out, void *stream);
+
+/* ---- fused batched Heston engine ------------------------------------------------
+ * One engine steps many options (uneven grids) with one launch per ADI stage;
+ * it replaces the per-step operator-call loops of
+ * FiniteDifferenceEngineGPU.pyx:379-593 and generates every coefficient and
+ * boundary row on the device from the 1-D meshes. */
+typedef struct fdadi_hb fdadi_hb;
+typedef struct {
+  double r, kappa, theta, sigma, rho; /* Heston parameters */
+  double dt;                          /* step size of this option */
+  int64_t nS, nv;                     /* grid size */
+  int32_t barrier;                    /* nonzero: Dirichlet 0 at S = Smax (up-and-out) */
+  int32_t pad;
+} fdadi_heston_option;
+/* spots/vars are the meshes of all options concatenated (host). */
+int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double *spots,
+                    const double *vars, void *stream, fdadi_hb **out);
+int fdadi_hb_destroy(fdadi_hb *hb);
+/* Select the scheme, theta and a multiplier applied to every option's dt;
+ * rebuilds the (time-invariant) factorisations on the device. */
+int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale,
+                       void *stream);
+/* State of option k lives at d_state + offset[k] (layout (nS,nv) row-major). */
+int fdadi_hb_state(fdadi_hb *hb, double **d_state, int64_t *total);
+int fdadi_hb_offsets(const fdadi_hb *hb, int64_t *offsets);
+/* Advance every option by nsteps steps of the configured scheme. */
+int fdadi_hb_step(fdadi_hb *hb, int32_t nsteps, void *stream);
+/* Capture nsteps steps into a HIP graph (replaces any previous one) and
+ * replay it. */
+int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream);
+int fdadi_hb_replay(fdadi_hb *hb, void *stream);
+/* Launch shape per sweep kind (0 spot, 1 var): threads per workgroup and
+ * lines per tile; used by the autotuner.  0 = library default. */
+int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines);
+int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines);
+/* Per-stage launch list for profiling: number of kernel launches per step. */
+int fdadi_hb_launches_per_step(const fdadi_hb *hb, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDADI_H */
