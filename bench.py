@@ -1,0 +1,238 @@
+#!/usr/bin/env python
+"""Throughput of the MI355X Heston ADI engine (one GPU).
+
+Default workload (the line's ``value``): a single FP64 Heston European call
+on a 2048 x 1024 (spot x variance) grid, rho = -0.7, Hundsdorfer-Verwer
+theta = 0.5, nt = 2000 steps (S0 = K = 100, r = 0.06, kappa = 1,
+theta = v0 = 0.04, sigma = 0.2, spot mesh on [0, 5K], variance mesh on
+[0, 5 theta]).  One bench "step" is one full option solve (2000 ADI steps) on
+device-resident inputs; ``value`` is options/s over the K timed solves.
+
+The line also carries
+* ``roofline``: the dominant kernel's algorithmic bytes per launch over its
+  HIP-event-measured average duration (events on the engine's stream, inside
+  the timed region), against the 8 TB/s HBM peak; ``traffic`` is read from
+  the rocprofv3 counter summary committed under profiles/ when present;
+* ``cpu_baseline``: the NumPy/SciPy restatement of the reference CPU engine
+  (oracle/), one thread, on a bounded sample of the same workload;
+* ``extra``: the 150 x 150 single option (plain loop) and the 4096-option
+  batched Craig-Sneyd workload in options/s.
+"""
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("options/sec at ≤1e-4 abs err vs FP64 ADI: surrogate+fallback vs batched "
+          "vs plain")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per grid point for each HV stage (reads + writes of state
+# vectors; coefficients and factors are regenerated and count 0).
+HV_STAGE_BYTES = [16, 16, 24, 16]
+HV_STAGE_NAMES = ["spot_sweep_kernel<SM_FIRST>", "var_sweep_kernel<VM_PLAIN>",
+                  "spot_sweep_kernel<SM_HV>", "var_sweep_kernel<VM_PLAIN>"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def config1():
+    from fd_adi_amd.workloads import single
+    return single(2048, 1024, 2000, -0.7)
+
+
+def run_single(opt, scheme, theta, K, W, smoothing=0):
+    """Time K solves of one option (state reset from a device-resident copy)."""
+    import numpy as np
+    import fd_adi_amd
+    hb = fd_adi_amd.HestonBatch([opt.params], [opt.spots], [opt.vars], [opt.dt])
+    U0 = opt.payoff()
+    hb.set_state([U0])
+    init = hb.state_view(0).copy(True)          # payoff kept in HBM
+    view = hb.state_view(0)
+
+    def solve(timed=False):
+        view.copy_from(init)
+        stages = None
+        if smoothing:
+            hb.configure("implicit", 1.0, 0.5)
+            hb.step(2 * smoothing)
+        hb.configure(scheme, theta, 1.0)
+        n = opt.nt - smoothing
+        if timed:
+            stages = hb.step_timed(n)
+        else:
+            hb.step(n)
+        return stages
+
+    for w in range(W):
+        solve(timed=(w == 0))  # also creates the event pool outside the timed region
+    hb.sync()
+    t0 = time.perf_counter()
+    stages = None
+    for k in range(K):
+        st = solve(timed=(k == K - 1))
+        if st is not None:
+            stages = st
+    hb.sync()
+    wall = time.perf_counter() - t0
+    price = hb.get_state()[0][opt.idx]
+    return wall, stages, price, hb
+
+
+def batched(n_opt, seed, K, W):
+    import numpy as np
+    import fd_adi_amd
+    from fd_adi_amd.workloads import batch_draws
+    t = time.perf_counter()
+    opts = batch_draws(n_opt, seed)
+    log("batch: %d options drawn in %.1fs" % (n_opt, time.perf_counter() - t))
+    hb = fd_adi_amd.HestonBatch([o.params for o in opts], [o.spots for o in opts],
+                                [o.vars for o in opts], [o.dt for o in opts])
+    hb.set_state([o.payoff() for o in opts])
+    init = hb.state_view().copy(True)
+    view = hb.state_view()
+    hb.configure("cs", 0.5, 1.0)
+    hb.step(2)
+    hb.capture(200)
+    for _ in range(W):
+        view.copy_from(init)
+        hb.replay()
+    hb.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        view.copy_from(init)
+        hb.replay()
+    hb.sync()
+    wall = time.perf_counter() - t0
+    npts = sum(o.spots.size * o.vars.size for o in opts)
+    return n_opt * K / wall, wall / K, npts
+
+
+def cpu_baseline(steps):
+    """Oracle (NumPy/SciPy restatement of the reference CPU engine), one
+    thread, ``steps`` HV steps of the 2048x1024 option; extrapolated to the
+    full 2000-step solve."""
+    code = r"""
+import os, sys, time, json
+sys.path.insert(0, %r)
+from fd_adi_amd.workloads import single
+from oracle.engine import HestonOracle, HestonParams
+o = single(2048, 1024, 2000, -0.7)
+E = HestonOracle(HestonParams(spot=o.S0, strike=o.K, r=o.r, v0=o.v0, T=o.T, kappa=o.kappa,
+                 theta=o.theta, sigma=o.sigma, rho=o.rho), o.spots, o.vars)
+U0 = o.payoff()
+E.solve_hundsdorferverwer(1, o.dt, U0)
+t = time.perf_counter()
+E.solve_hundsdorferverwer(%d, o.dt, U0)
+print(json.dumps({"sec_per_step": (time.perf_counter() - t) / %d}))
+""" % (ROOT, steps, steps)
+    env = dict(os.environ)
+    for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS",
+              "NUMEXPR_NUM_THREADS", "BLIS_NUM_THREADS"):
+        env[v] = "1"
+    env["HIP_VISIBLE_DEVICES"] = ""
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError(out.stderr[-2000:])
+    sec = json.loads(out.stdout.strip().splitlines()[-1])["sec_per_step"]
+    return sec
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_config1.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*%s\s+bytes_per_launch\s*=\s*([0-9.eE+]+)" % re.escape(kernel), line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed option solves")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
+              "(options shard across GPUs with no exchange; see the design notes)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import fd_adi_amd  # noqa: F401
+
+    opt = config1()
+    N = opt.spots.size * opt.vars.size
+    log("config1: %dx%d, nt=%d, HV; warmup %d, timed %d" % (opt.spots.size, opt.vars.size,
+                                                           opt.nt, args.warmup, args.steps))
+    wall, stages, price, _ = run_single(opt, "hv", 0.5, args.steps, args.warmup)
+    ms_per = 1e3 * wall / args.steps
+    value = args.steps / wall
+    per_launch_ms = stages / opt.nt
+    stage_gbs = [HV_STAGE_BYTES[k] * N / (per_launch_ms[k] * 1e-3) / 1e9 for k in range(4)]
+    dom = int(np.argmax(stages))
+    achieved = stage_gbs[dom]
+    log("config1: %.2f ms/option, price %.12f; stage ms/launch %s; GB/s %s" % (
+        ms_per, price, np.round(per_launch_ms, 4).tolist(), np.round(stage_gbs, 1).tolist()))
+    algo_step = sum(HV_STAGE_BYTES) * N
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(HV_STAGE_NAMES[dom]),
+                "kernel": HV_STAGE_NAMES[dom], "bytes_per_launch": HV_STAGE_BYTES[dom] * N,
+                "avg_launch_us": round(per_launch_ms[dom] * 1e3, 3),
+                "stage_GBs": [round(x, 1) for x in stage_gbs],
+                "step_GBs": round(algo_step / (wall / args.steps / opt.nt) / 1e9, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: %d HV steps of the same option on one core" % args.cpu_sample_steps)
+        sec = cpu_baseline(args.cpu_sample_steps)
+        cpu = {"value": 1.0 / (sec * opt.nt), "unit": "options/s", "cores": 1, "kind": "port",
+               "sample": "%d of 2000 HV steps of the 2048x1024 option (oracle, NumPy/SciPy "
+                         "dia.dot + LAPACK solve_banded), extrapolated" % args.cpu_sample_steps,
+               "sec_per_step": sec}
+
+    extra = {}
+    if not args.no_extra:
+        from fd_adi_amd.workloads import single
+        o0 = single(150, 150, 150, 0.0, spot_max=500.0, var_max=0.2)
+        w0, _, p0, _ = run_single(o0, "douglas", 0.5, 20, 2, smoothing=2)
+        extra["plain_150x150_douglas_smoothed_options_per_s"] = round(20 / w0, 2)
+        extra["plain_150x150_price"] = p0
+        rate, sec, npts = batched(4096, 2024, 2, 1)
+        extra["batched_4096_cs_options_per_s"] = round(rate, 1)
+        extra["batched_4096_cs_sec"] = round(sec, 4)
+        extra["batched_4096_cs_GBs_algorithmic"] = round(72 * npts * 200 / sec / 1e9, 1)
+        log("extra: %s" % extra)
+
+    line = {
+        "metric": METRIC, "value": round(value, 4), "unit": "options/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "heston_call_2048x1024_hv_nt2000_rho-0.7", "nS": 2048,
+                   "nv": 1024, "nt": 2000, "scheme": "hundsdorferverwer", "theta": 0.5,
+                   "price": price},
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,6 +1,6 @@
 """ctypes binding of libfdadi.so (the C ABI declared in include/fdadi.h).
 
-The library must have been built (``python -m fd_adi_amd.build``); there is
+The library must have been built (``python fd_adi_amd/build.py``); there is
 no fallback: importing the package without it raises ImportError.
 """
 
@@ -16,7 +16,7 @@ SCHEME_IMPLICIT, SCHEME_DOUGLAS, SCHEME_CS, SCHEME_MCS, SCHEME_HV = 0, 1, 2, 3, 
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("fd_adi_amd: %s is missing; build it with "
-                      "`python -m fd_adi_amd.build` (no CPU fallback exists)" % LIB_PATH)
+                      "`python fd_adi_amd/build.py` (no CPU fallback exists)" % LIB_PATH)
 
 lib = ctypes.CDLL(LIB_PATH)
 
@@ -97,15 +97,23 @@ _SIGS = {
     "fdadi_hb_state": (c_int, [c_vp, _P(c_vp), _P(c_i64)]),
     "fdadi_hb_offsets": (c_int, [c_vp, c_vp]),
     "fdadi_hb_step": (c_int, [c_vp, c_i32, c_vp]),
+    "fdadi_hb_step_timed": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i32, _P(c_i32)]),
     "fdadi_hb_capture": (c_int, [c_vp, c_i32, c_vp]),
     "fdadi_hb_replay": (c_int, [c_vp, c_vp]),
     "fdadi_hb_set_shape": (c_int, [c_vp, c_int, c_int, c_int]),
     "fdadi_hb_get_shape": (c_int, [c_vp, c_int, _P(c_int), _P(c_int)]),
     "fdadi_hb_launches_per_step": (c_int, [c_vp, _P(c_int)]),
+    "fdadi_mlp_nparam": (c_int, [_P(c_i64)]),
+    "fdadi_mlp_create": (c_int, [c_i32, c_vp, c_i64, c_vp, c_vp, c_double, c_double, _P(c_vp)]),
+    "fdadi_mlp_destroy": (c_int, [c_vp]),
+    "fdadi_mlp_get_params": (c_int, [c_vp, c_vp, c_vp]),
+    "fdadi_mlp_set_params": (c_int, [c_vp, c_vp, c_vp]),
+    "fdadi_mlp_grad": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "fdadi_mlp_train_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, ctypes.c_float, c_vp]),
+    "fdadi_mlp_loss": (c_int, [c_vp, c_vp]),
+    "fdadi_mlp_predict": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "fdadi_mlp_gemm_nt": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
 }
-
-_MLP_SIGS = {}
-
 
 def _declare(table):
     for name, (res, args) in table.items():

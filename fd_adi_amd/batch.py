@@ -123,6 +123,15 @@ class HestonBatch(object):
             raise RuntimeError("HestonBatch.step: call configure() first")
         _lib.call("fdadi_hb_step", self._h, int(nsteps), self.stream)
 
+    def step_timed(self, nsteps):
+        """Step with HIP events around every stage; returns the summed device
+        milliseconds per stage slot (launch order within a step)."""
+        out = np.zeros(8)
+        n = ctypes.c_int32()
+        _lib.call("fdadi_hb_step_timed", self._h, int(nsteps), self.stream, _lib.ptr(out), 8,
+                  ctypes.byref(n))
+        return out[:n.value].copy()
+
     def capture(self, nsteps):
         _lib.call("fdadi_hb_capture", self._h, int(nsteps), self.stream)
 

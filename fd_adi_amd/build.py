@@ -1,6 +1,7 @@
 """Build libfdadi.so (HIP, gfx950) in-tree.
 
-    python -m fd_adi_amd.build          # or fd_adi_amd.build.build()
+    python fd_adi_amd/build.py      (run as a script: importing the package
+                                     would load the library being rebuilt)
 
 The library is a single shared object with a C ABI (include/fdadi.h); the
 Python layer loads it with ctypes.  Sources are compiled separately and

@@ -78,6 +78,7 @@ struct Ctx {
   const double *vt;  // variance tables, field-major, stride vt_n
   int64_t sp_n, vt_n;
   const double *rd0;  // spot-line pivots 1/u, state layout
+  const double *spr;  // spot table, one 16-double row per node
 };
 
 __device__ __forceinline__ double SP(const Ctx &c, const HOpt &o, int f, int64_t i) {
@@ -238,7 +239,80 @@ __device__ void affine_scan(const double *cM, const double *cE, double *cIn, int
 }
 
 // ---- spot sweep ---------------------------------------------------------------
-// LDS: d[nSmax * NL] (row i, column jj at i*NL+jj) + 3 * TPL * NL carries.
+// One 128-byte row of per-node spot coefficients (AoS copy of the spot table):
+// [psub pmid psup qsub qmid qsup pr qr wsub wmid wsup S dir - - -].
+constexpr int kSpRow = 16;
+
+struct SpotRow {
+  double psub, pmid, psup, qsub, qmid, qsup, pr, qr, wsub, wmid, wsup, S, dir;
+};
+
+__device__ __forceinline__ SpotRow load_spot_row(const double *__restrict__ spr) {
+  const double2 *p = reinterpret_cast<const double2 *>(spr);
+  const double2 a = p[0], b = p[1], c2 = p[2], d = p[3], e = p[4], f = p[5], g = p[6];
+  SpotRow r;
+  r.psub = a.x; r.pmid = a.y; r.psup = b.x; r.qsub = b.y; r.qmid = c2.x; r.qsup = c2.y;
+  r.pr = d.x; r.qr = d.y; r.wsub = e.x; r.wmid = e.y; r.wsup = f.x; r.S = f.y; r.dir = g.x;
+  return r;
+}
+
+// 3x3 neighbourhood of a field around (i, j): rows i-1, i, i+1; cols j-1, j, j+1.
+struct Win {
+  double v[3][3];
+};
+
+__device__ __forceinline__ void load_row(const double *__restrict__ f, int64_t i, int64_t nS,
+                                         int64_t nv, int64_t jm, int64_t j, int64_t jp,
+                                         double out[3]) {
+  const int64_t ii = i < 0 ? 0 : (i >= nS ? nS - 1 : i);
+  const double *r = f + ii * nv;
+  out[0] = r[jm];
+  out[1] = r[j];
+  out[2] = r[jp];
+}
+
+struct VarCol {  // variance coefficients of column j (constant along a spot line)
+  double gsub, gmid, gsup, wv0, wv1, wv2, v;
+};
+
+// (A0 f)(i,j) from the window; 0 on Dirichlet rows, neighbours on Dirichlet
+// rows forced to 0.
+__device__ __forceinline__ double a0w(const SpotRow &R, double v, const Win &w, bool dm,
+                                      bool dp) {
+  if (R.dir != 0.0) return 0.0;
+  const double sub = R.psub + v * R.qsub, mid = R.pmid + v * R.qmid, sup = R.psup + v * R.qsup;
+  return sub * (dm ? 0.0 : w.v[0][1]) + mid * w.v[1][1] + sup * (dp ? 0.0 : w.v[2][1]);
+}
+
+__device__ __forceinline__ double a1w(const HOpt &o, const SpotRow &R, const VarCol &q,
+                                      const Win &w, bool last, double fjm2) {
+  const double snz = R.S != 0.0 ? 1.0 : 0.0;
+  double acc = (snz * q.gmid - 0.5 * o.r) * w.v[1][1];
+  acc += snz * (q.gsub * w.v[1][0] + q.gsup * w.v[1][2] + (last ? o.Gz * fjm2 : 0.0));
+  return acc;
+}
+
+__device__ __forceinline__ double a01w(const HOpt &o, const SpotRow &R, const VarCol &q,
+                                       const Win &w) {
+  const double r0 = q.wv0 * w.v[0][0] + q.wv1 * w.v[0][1] + q.wv2 * w.v[0][2];
+  const double r1 = q.wv0 * w.v[1][0] + q.wv1 * w.v[1][1] + q.wv2 * w.v[1][2];
+  const double r2 = q.wv0 * w.v[2][0] + q.wv1 * w.v[2][1] + q.wv2 * w.v[2][2];
+  const double coef = ((o.rho * o.sigma) * R.S) * q.v;
+  return coef * (R.wsub * r0 + R.wmid * r1 + R.wsup * r2);
+}
+
+// Work-group index -> tile index so that consecutive tiles (which share cache
+// lines of the state) run on the same XCD: dispatch places work-group k on
+// XCD k mod 8.
+__device__ __forceinline__ int xcd_tile(int k, int G) {
+  const int q = G / 8, rem = G % 8, x = k % 8, y = k / 8;
+  return x * q + (x < rem ? x : rem) + y;
+}
+
+// A spot sweep: one work-group per (option, NL adjacent variance columns);
+// thread (jj, ch) owns rows [ch*L, ch*L+L) of column j0+jj through every
+// phase.  LDS: d[nSmax*NL] (right-hand side, then the solution),
+// e[nSmax*NL] (explicit term of the next stage), 3*TPL*NL carries.
 template <int MODE, int THREADS, int NL, int MAXL>
 __global__ void __launch_bounds__(THREADS)
 spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ U,
@@ -246,48 +320,112 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   extern __shared__ double lds[];
   constexpr int TPL = THREADS / NL;
   double *d = lds;
-  double *cM = lds + nSmax * NL;
+  double *eb = lds + nSmax * NL;
+  double *cM = eb + nSmax * NL;
   double *cE = cM + TPL * NL;
   double *cIn = cE + TPL * NL;
-  const Tile t = tiles[blockIdx.x];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const HOpt o = c.opts[t.opt];
   const int64_t nS = o.nS, nv = o.nv, j0 = t.start;
   const int tid = threadIdx.x;
-
-  // Phase 1: fused right-hand side into LDS.
-  for (int64_t q = tid; q < nS * NL; q += THREADS) {
-    const int64_t i = q / NL, j = j0 + q % NL;
-    // Dirichlet rows: the solve forces the right-hand side to the value 0.
-    d[q] = (j < nv && SP(c, o, SP_DIR, i) == 0.0) ? spot_rhs<MODE>(c, o, i, j, U, Y) : 0.0;
-  }
-  __syncthreads();
-
-  // Phase 2: chunked LU solve of (I - tdt A0) x = rhs on each column.
   const int jj = tid % NL, ch = tid / NL;
   const int64_t j = j0 + jj;
   const int L = (int)((nS + TPL - 1) / TPL);
   const int64_t s = (int64_t)ch * L;
   const int64_t e = s + L < nS ? s + L : nS;
   const bool active = j < nv && s < e;
-  const double v = j < nv ? VT(c, o, VT_V, j) : 0.0;
+  const double *u = U + o.off;
+  const double *y2 = (MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS) ? Y + o.off : nullptr;
   const double *rd = c.rd0 + o.off;
+  const double *spr = c.spr + o.so * kSpRow;
   double mreg[MAXL], rdreg[MAXL];
   double M = 1.0, E = 0.0;
+
+  // Phase 1: walk the chunk with a 3x3 window, form the fused right-hand side
+  // (and the next stage's explicit term), and run the local forward recurrence.
   if (active) {
+    VarCol q;
+    q.gsub = VT(c, o, VT_GSUB, j); q.gmid = VT(c, o, VT_GMID, j); q.gsup = VT(c, o, VT_GSUP, j);
+    q.wv0 = VT(c, o, VT_WSUB, j); q.wv1 = VT(c, o, VT_WMID, j); q.wv2 = VT(c, o, VT_WSUP, j);
+    q.v = VT(c, o, VT_V, j);
+    const int64_t jm = j > 0 ? j - 1 : 0, jp = j < nv - 1 ? j + 1 : nv - 1;
+    const bool last = j == nv - 1;
+    const double tdt = o.tdt, dt = o.dt;
+    Win wu, wy;
+    load_row(u, s - 1, nS, nv, jm, j, jp, wu.v[0]);
+    load_row(u, s, nS, nv, jm, j, jp, wu.v[1]);
+    load_row(u, s + 1, nS, nv, jm, j, jp, wu.v[2]);
+    if (y2) {
+      load_row(y2, s - 1, nS, nv, jm, j, jp, wy.v[0]);
+      load_row(y2, s, nS, nv, jm, j, jp, wy.v[1]);
+      load_row(y2, s + 1, nS, nv, jm, j, jp, wy.v[2]);
+    }
     double rdprev = s > 0 ? rd[(s - 1) * nv + j] : 0.0;
 #pragma unroll
     for (int k = 0; k < MAXL; ++k) {
       const int64_t i = s + k;
       if (i < e) {
-        double a, b, cc;
-        li0_row(c, o, i, v, a, b, cc);
-        const double m = -a * rdprev;
+        const SpotRow R = load_spot_row(spr + i * kSpRow);
+        const bool dm = i == 1, dp = o.barrier && i + 1 == nS - 1;
+        const double uj2 = last ? u[i * nv + j - 2] : 0.0;
+        double rhs = 0.0, ex = 0.0;
+        if (MODE == SM_IMPL) {
+          rhs = wu.v[1][1] + dt * (R.pr + q.v * R.qr);
+        } else {
+          const double a0u = a0w(R, q.v, wu, dm, dp);
+          const double a1u = a1w(o, R, q, wu, last, uj2);
+          const double r0 = R.pr + q.v * R.qr;  // zero on Dirichlet rows
+          const double fu = dt * (a0u + r0) + dt * a1u + dt * a01w(o, R, q, wu);
+          rhs = wu.v[1][1] + fu;
+          if (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) ex = tdt * a1u;
+          if (MODE == SM_FIRST) {
+            rhs -= tdt * a0u;
+          } else {
+            // Operators are linear: A (Y2 - U) = A Y2 - A U.
+            const double yj2 = last ? y2[i * nv + j - 2] : 0.0;
+            const double a0y = a0w(R, q.v, wy, dm, dp);
+            const double a1y = a1w(o, R, q, wy, last, yj2);
+            const double c01 = a01w(o, R, q, wy) - a01w(o, R, q, wu);
+            if (MODE == SM_HV) {
+              const double fd = dt * (a0y - a0u) + dt * (a1y - a1u) + dt * c01;
+              rhs += 0.5 * fd - tdt * a0y;
+              ex = tdt * a1y;
+            } else if (MODE == SM_CS) {
+              rhs += (0.5 * dt) * c01 - tdt * a0u;
+            } else {  // SM_MCS
+              const double th = tdt / dt;
+              const double fd = dt * ((a0y - a0u) + r0) + dt * (a1y - a1u) + dt * c01;
+              rhs += (th * dt) * c01 + (0.5 - th) * fd - tdt * a0u;
+            }
+          }
+        }
+        if (R.dir != 0.0) rhs = 0.0;  // the solve forces Dirichlet rows to 0
+        d[i * NL + jj] = rhs;
+        eb[i * NL + jj] = ex;
+        // I - tdt*A0 on this row; pivots from the factorisation.
+        const double aL = (R.dir != 0.0 || i == 0) ? 0.0 : -tdt * (R.psub + q.v * R.qsub);
+        const double m = -aL * rdprev;
         const double rdi = rd[i * nv + j];
         mreg[k] = m;
         rdreg[k] = rdi;
-        E = d[i * NL + jj] + m * E;
+        E = rhs + m * E;
         M = m * M;
         rdprev = rdi;
+        // advance the windows by one row
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          wu.v[0][b] = wu.v[1][b];
+          wu.v[1][b] = wu.v[2][b];
+        }
+        load_row(u, i + 2, nS, nv, jm, j, jp, wu.v[2]);
+        if (y2) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            wy.v[0][b] = wy.v[1][b];
+            wy.v[1][b] = wy.v[2][b];
+          }
+          load_row(y2, i + 2, nS, nv, jm, j, jp, wy.v[2]);
+        }
       }
     }
   }
@@ -298,24 +436,25 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   __syncthreads();
   // Forward fix-up, then the local backward recurrence x = rd y + n x'.
   if (active) {
-    double y = cIn[ch * NL + jj];
+    double yv = cIn[ch * NL + jj];
 #pragma unroll
     for (int k = 0; k < MAXL; ++k) {
       const int64_t i = s + k;
       if (i < e) {
-        y = d[i * NL + jj] + mreg[k] * y;
-        d[i * NL + jj] = y;
+        yv = d[i * NL + jj] + mreg[k] * yv;
+        d[i * NL + jj] = yv;
       }
     }
+    const double v = VT(c, o, VT_V, j);
     M = 1.0;
     E = 0.0;
 #pragma unroll
     for (int k = MAXL - 1; k >= 0; --k) {
       const int64_t i = s + k;
       if (i < e) {
-        double a, b, cc;
-        li0_row(c, o, i, v, a, b, cc);
-        const double n = -cc * rdreg[k];
+        const double *rw = spr + i * kSpRow;
+        const double cL = (rw[12] != 0.0 || i == nS - 1) ? 0.0 : -o.tdt * (rw[2] + v * rw[5]);
+        const double n = -cL * rdreg[k];
         mreg[k] = n;
         E = rdreg[k] * d[i * NL + jj] + n * E;
         M = n * M;
@@ -341,11 +480,12 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   }
   __syncthreads();
 
-  // Phase 3: fused stage output.
+  // Phase 3: stage output x - tdt*A1(f), written with consecutive lanes on
+  // consecutive columns.
   double *ob = out + o.off;
-  for (int64_t q = tid; q < nS * NL; q += THREADS) {
-    const int64_t i = q / NL, jq = j0 + q % NL;
-    if (jq < nv) ob[i * nv + jq] = spot_out<MODE>(c, o, i, jq, d[q], U, Y);
+  for (int64_t qi = tid; qi < nS * NL; qi += THREADS) {
+    const int64_t i = qi / NL, jq = j0 + qi % NL;
+    if (jq < nv) ob[i * nv + jq] = d[qi] - eb[qi];
   }
 }
 
@@ -471,7 +611,8 @@ struct SetupOpt {
 // One thread per spot node of every option: the (0,) and (0,0) template rows
 // (with boundary rows) scaled by r*S and 1/2*S^2, and the cross weights.
 __global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *row_opt,
-                                   const double *__restrict__ spots, double *sp, int64_t sp_n) {
+                                   const double *__restrict__ spots, double *sp, int64_t sp_n,
+                                   double *spr) {
 #pragma clang fp contract(off)
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (g >= sp_n) return;
@@ -515,6 +656,9 @@ __global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *ro
   col[SP_WSUP * sp_n] = ws[2];
   col[SP_S * sp_n] = S;
   col[SP_DIR * sp_n] = dir ? 1.0 : 0.0;
+  double *row = spr + g * 16;
+  for (int f = 0; f < SP_NF; ++f) row[f] = col[f * sp_n];
+  for (int f = SP_NF; f < 16; ++f) row[f] = 0.0;
 }
 
 // One thread per variance node: A1 rows for S != 0 lines and cross weights.
@@ -645,7 +789,7 @@ struct fdadi_hb {
   std::vector<double> base_dt;
   HOpt *d_opts = nullptr;
   double *d_spots = nullptr, *d_vars = nullptr;
-  double *d_sp = nullptr, *d_vt = nullptr;
+  double *d_sp = nullptr, *d_vt = nullptr, *d_spr = nullptr;
   int64_t sp_n = 0, vt_n = 0, total = 0;
   int64_t nSmax = 0, nvmax = 0;
   double *d_state = nullptr, *d_w1 = nullptr, *d_y2 = nullptr, *d_rd0 = nullptr;
@@ -653,17 +797,21 @@ struct fdadi_hb {
   Tile *d_stiles = nullptr, *d_vtiles = nullptr, *d_ftiles = nullptr;
   int nstiles = 0, nvtiles = 0, nftiles = 0;
   int spot_threads = 0, var_threads = 0;
+  int spot_maxl = 8;  // rows per thread in the spot sweep (8, or 16 for long lines)
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  std::vector<hipEvent_t> events;  // pool for fdadi_hb_step_timed
 };
 
 namespace fdadi {
 namespace {
 
-constexpr int kNL = 8;
-constexpr int kMAXL = 16;
+constexpr int kNL = 8;     // variance sweep: rows per tile
+constexpr int kMAXL = 16;  // variance sweep: positions per thread
+constexpr int kNLs = 4;    // spot sweep: columns per tile
+constexpr int kMAXLs = 8;  // spot sweep: rows per thread
 
 Ctx ctx_of(const fdadi_hb *hb) {
   Ctx c;
@@ -673,6 +821,7 @@ Ctx ctx_of(const fdadi_hb *hb) {
   c.sp_n = hb->sp_n;
   c.vt_n = hb->vt_n;
   c.rd0 = hb->d_rd0;
+  c.spr = hb->d_spr;
   return c;
 }
 
@@ -681,6 +830,36 @@ Ctx ctx_of(const fdadi_hb *hb) {
     int _st = (x);           \
     if (_st) return _st;     \
   } while (0)
+
+// Spot-sweep shape: `threads` (0 = default) and rows per thread for lines of
+// length n.  Lines up to 1024 use 8 rows per thread; longer lines 16 rows per
+// thread at 512 threads (1024 x 8 would exceed the 128-VGPR budget).
+int spot_shape(int64_t n, int threads, int *t_out, int *maxl_out) {
+  if (threads == 0) {
+    if (n <= 1024) {
+      const int64_t need = (n + kMAXLs - 1) / kMAXLs * kNLs;
+      int t = 64;
+      while (t < need) t *= 2;
+      *t_out = t;
+      *maxl_out = kMAXLs;
+    } else {
+      *t_out = 512;
+      *maxl_out = 2 * kMAXLs;
+    }
+    return FDADI_OK;
+  }
+  if ((int64_t)(threads / kNLs) * kMAXLs >= n && threads <= 1024) {
+    *t_out = threads;
+    *maxl_out = kMAXLs;
+  } else if ((int64_t)(threads / kNLs) * 2 * kMAXLs >= n && (threads == 256 || threads == 512)) {
+    *t_out = threads;
+    *maxl_out = 2 * kMAXLs;
+  } else {
+    set_error("fdadi_hb_set_shape: too few threads for the longest spot line");
+    return FDADI_ERR_ARG;
+  }
+  return FDADI_OK;
+}
 
 // Kernels using more than 64 KB of dynamic LDS must say so once.
 int allow_lds(const void *fn, size_t bytes) {
@@ -691,21 +870,21 @@ int allow_lds(const void *fn, size_t bytes) {
   return FDADI_OK;
 }
 
-int threads_for(int64_t n) {
+int threads_for(int64_t n, int nl = kNL, int maxl = kMAXL) {
   // Threads per workgroup so that each line's chunk fits MAXL registers.
-  const int64_t need = (n + kMAXL - 1) / kMAXL * kNL;
+  const int64_t need = (n + maxl - 1) / maxl * nl;
   int t = 64;
   while (t < need && t < 1024) t *= 2;
   return t;
 }
 
-template <int MODE, int T>
+template <int MODE, int T, int ML = kMAXLs>
 int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                   hipStream_t s) {
-  constexpr int TPL = T / kNL;
-  const size_t lds = sizeof(double) * (hb->nSmax * kNL + 3 * TPL * kNL);
-  FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, kNL, kMAXL>, lds));
-  hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, kNL, kMAXL>), dim3(hb->nstiles), dim3(T), lds,
+  constexpr int TPL = T / kNLs;
+  const size_t lds = sizeof(double) * (2 * hb->nSmax * kNLs + 3 * TPL * kNLs);
+  FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, kNLs, ML>, lds));
+  hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, kNLs, ML>), dim3(hb->nstiles), dim3(T), lds,
                      s, ctx_of(hb), hb->d_stiles, U, Y, out, hb->nSmax);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
@@ -714,6 +893,14 @@ int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *
 template <int MODE>
 int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                 hipStream_t s) {
+  if (hb->spot_maxl == 2 * kMAXLs) {
+    switch (hb->spot_threads) {
+      case 256: return launch_spot_t<MODE, 256, 2 * kMAXLs>(hb, U, Y, out, s);
+      case 512: return launch_spot_t<MODE, 512, 2 * kMAXLs>(hb, U, Y, out, s);
+    }
+    set_error("spot sweep: unsupported thread count for 16 rows per thread");
+    return FDADI_ERR_UNSUPPORTED;
+  }
   switch (hb->spot_threads) {
     case 64: return launch_spot_t<MODE, 64>(hb, U, Y, out, s);
     case 128: return launch_spot_t<MODE, 128>(hb, U, Y, out, s);
@@ -750,35 +937,54 @@ int launch_var(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) 
   return FDADI_ERR_UNSUPPORTED;
 }
 
+// Optional per-stage event pairs (fdadi_hb_step_timed); null = untimed.
+struct StageEvents {
+  hipEvent_t *ev;  // 2 per stage, in launch order
+  int next;
+};
+
+int mark(StageEvents *t, hipStream_t s) {
+  if (!t) return FDADI_OK;
+  FDADI_TRY_HIP(hipEventRecord(t->ev[t->next++], s));
+  return FDADI_OK;
+}
+
+#define STAGE(call)                 \
+  do {                              \
+    FDADI_RET(mark(tm, s));         \
+    FDADI_RET(call);                \
+    FDADI_RET(mark(tm, s));         \
+  } while (0)
+
 // One time step of the configured scheme (stage list of the module comment).
-int step_once(fdadi_hb *hb, hipStream_t s) {
+int step_once(fdadi_hb *hb, hipStream_t s, StageEvents *tm = nullptr) {
   double *U = hb->d_state, *W = hb->d_w1, *Y2 = hb->d_y2;
   switch (hb->scheme) {
     case FDADI_SCHEME_DOUGLAS:
-      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
-      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      STAGE((launch_spot<SM_FIRST>(hb, U, nullptr, W, s)));
+      STAGE((launch_var<VM_PLAIN>(hb, W, U, s)));
       return FDADI_OK;
     case FDADI_SCHEME_IMPLICIT:
-      FDADI_RET(launch_var<VM_IMPL>(hb, U, W, s));
-      FDADI_RET(launch_spot<SM_IMPL>(hb, W, nullptr, U, s));
+      STAGE((launch_var<VM_IMPL>(hb, U, W, s)));
+      STAGE((launch_spot<SM_IMPL>(hb, W, nullptr, U, s)));
       return FDADI_OK;
     case FDADI_SCHEME_HV:
-      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
-      FDADI_RET(launch_var<VM_PLAIN>(hb, W, Y2, s));
-      FDADI_RET(launch_spot<SM_HV>(hb, U, Y2, W, s));
-      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      STAGE((launch_spot<SM_FIRST>(hb, U, nullptr, W, s)));
+      STAGE((launch_var<VM_PLAIN>(hb, W, Y2, s)));
+      STAGE((launch_spot<SM_HV>(hb, U, Y2, W, s)));
+      STAGE((launch_var<VM_PLAIN>(hb, W, U, s)));
       return FDADI_OK;
     case FDADI_SCHEME_CS:
-      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
-      FDADI_RET(launch_var<VM_PLAIN>(hb, W, Y2, s));
-      FDADI_RET(launch_spot<SM_CS>(hb, U, Y2, W, s));
-      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      STAGE((launch_spot<SM_FIRST>(hb, U, nullptr, W, s)));
+      STAGE((launch_var<VM_PLAIN>(hb, W, Y2, s)));
+      STAGE((launch_spot<SM_CS>(hb, U, Y2, W, s)));
+      STAGE((launch_var<VM_PLAIN>(hb, W, U, s)));
       return FDADI_OK;
     case FDADI_SCHEME_MCS:
-      FDADI_RET(launch_spot<SM_FIRST>(hb, U, nullptr, W, s));
-      FDADI_RET(launch_var<VM_PLAIN>(hb, W, Y2, s));
-      FDADI_RET(launch_spot<SM_MCS>(hb, U, Y2, W, s));
-      FDADI_RET(launch_var<VM_PLAIN>(hb, W, U, s));
+      STAGE((launch_spot<SM_FIRST>(hb, U, nullptr, W, s)));
+      STAGE((launch_var<VM_PLAIN>(hb, W, Y2, s)));
+      STAGE((launch_spot<SM_MCS>(hb, U, Y2, W, s)));
+      STAGE((launch_var<VM_PLAIN>(hb, W, U, s)));
       return FDADI_OK;
   }
   set_error("fdadi_hb_step: engine not configured (call fdadi_hb_configure)");
@@ -789,7 +995,8 @@ void free_hb(fdadi_hb *hb) {
   if (!hb) return;
   if (hb->gexec) (void)hipGraphExecDestroy(hb->gexec);
   if (hb->graph) (void)hipGraphDestroy(hb->graph);
-  void *ptrs[] = {hb->d_opts, hb->d_spots, hb->d_vars, hb->d_sp, hb->d_vt, hb->d_state,
+  for (hipEvent_t e : hb->events) (void)hipEventDestroy(e);
+  void *ptrs[] = {hb->d_opts, hb->d_spots, hb->d_vars, hb->d_sp, hb->d_vt, hb->d_spr, hb->d_state,
                   hb->d_w1, hb->d_y2, hb->d_rd0, hb->d_sp_row, hb->d_vt_row, hb->d_stiles,
                   hb->d_vtiles, hb->d_ftiles};
   for (void *p : ptrs)
@@ -862,14 +1069,14 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   std::vector<Tile> st, vtl, ft;
   for (int k : order) {
     const HOpt &o = hb->h_opts[k];
-    for (int64_t j0 = 0; j0 < o.nv; j0 += kNL) st.push_back(Tile{k, (int32_t)j0});
+    for (int64_t j0 = 0; j0 < o.nv; j0 += kNLs) st.push_back(Tile{k, (int32_t)j0});
     for (int64_t i0 = 0; i0 < o.nS; i0 += kNL) vtl.push_back(Tile{k, (int32_t)i0});
     for (int64_t j0 = 0; j0 < o.nv; j0 += 64) ft.push_back(Tile{k, (int32_t)j0});
   }
   hb->nstiles = (int)st.size();
   hb->nvtiles = (int)vtl.size();
   hb->nftiles = (int)ft.size();
-  hb->spot_threads = threads_for(hb->nSmax);
+  spot_shape(hb->nSmax, 0, &hb->spot_threads, &hb->spot_maxl);
   hb->var_threads = threads_for(hb->nvmax);
   hipError_t e = hipSuccess;
   auto dalloc = [&](void **p, size_t bytes) {
@@ -880,6 +1087,7 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   dalloc((void **)&hb->d_vars, sizeof(double) * vo);
   dalloc((void **)&hb->d_sp, sizeof(double) * SP_NF * so);
   dalloc((void **)&hb->d_vt, sizeof(double) * VT_NF * vo);
+  dalloc((void **)&hb->d_spr, sizeof(double) * 16 * so);
   dalloc((void **)&hb->d_state, sizeof(double) * off);
   dalloc((void **)&hb->d_w1, sizeof(double) * off);
   dalloc((void **)&hb->d_y2, sizeof(double) * off);
@@ -902,7 +1110,7 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   up(hb->d_ftiles, ft.data(), sizeof(Tile) * ft.size());
   if (e == hipSuccess) {
     hipLaunchKernelGGL(spot_tables_kernel, dim3(grid_for(so, 256)), dim3(256), 0, s,
-                       hb->d_opts, nopt, hb->d_sp_row, hb->d_spots, hb->d_sp, so);
+                       hb->d_opts, nopt, hb->d_sp_row, hb->d_spots, hb->d_sp, so, hb->d_spr);
     hipLaunchKernelGGL(var_tables_kernel, dim3(grid_for(vo, 256)), dim3(256), 0, s, hb->d_opts,
                        hb->d_vt_row, hb->d_vars, hb->d_vt, vo);
     e = hipGetLastError();
@@ -980,6 +1188,35 @@ int fdadi_hb_step(fdadi_hb *hb, int32_t nsteps, void *stream) {
   return FDADI_OK;
 }
 
+int fdadi_hb_step_timed(fdadi_hb *hb, int32_t nsteps, void *stream, double *stage_ms,
+                        int32_t max_stages, int32_t *nstages) {
+  FDADI_REQUIRE(hb && stage_ms && nstages && nsteps > 0, FDADI_ERR_ARG,
+                "fdadi_hb_step_timed: bad arguments");
+  int per = 0;
+  FDADI_RET(fdadi_hb_launches_per_step(hb, &per));
+  FDADI_REQUIRE(per <= max_stages, FDADI_ERR_ARG, "fdadi_hb_step_timed: stage_ms too short");
+  hipStream_t s = as_stream(stream);
+  const size_t need = (size_t)2 * per * nsteps;
+  while (hb->events.size() < need) {
+    hipEvent_t e;
+    FDADI_TRY_HIP(hipEventCreate(&e));
+    hb->events.push_back(e);
+  }
+  StageEvents tm{hb->events.data(), 0};
+  for (int32_t n = 0; n < nsteps; ++n) FDADI_RET(step_once(hb, s, &tm));
+  FDADI_TRY_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < per; ++k) stage_ms[k] = 0.0;
+  for (int32_t n = 0; n < nsteps; ++n)
+    for (int k = 0; k < per; ++k) {
+      float ms = 0.f;
+      const size_t b = (size_t)2 * (n * per + k);
+      FDADI_TRY_HIP(hipEventElapsedTime(&ms, hb->events[b], hb->events[b + 1]));
+      stage_ms[k] += ms;
+    }
+  *nstages = per;
+  return FDADI_OK;
+}
+
 int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream) {
   FDADI_REQUIRE(hb, FDADI_ERR_ARG, "fdadi_hb_capture: null engine");
   FDADI_REQUIRE(nsteps > 0, FDADI_ERR_ARG, "fdadi_hb_capture: step count must be positive");
@@ -1020,15 +1257,20 @@ int fdadi_hb_replay(fdadi_hb *hb, void *stream) {
 
 int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
   FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_set_shape: bad kind");
-  FDADI_REQUIRE(lines == 0 || lines == kNL, FDADI_ERR_UNSUPPORTED,
-                "fdadi_hb_set_shape: lines per tile is fixed at 8");
-  const int64_t n = kind == 0 ? hb->nSmax : hb->nvmax;
-  int t = threads == 0 ? threads_for(n) : threads;
-  FDADI_REQUIRE(t == 64 || t == 128 || t == 256 || t == 512 || t == 1024, FDADI_ERR_ARG,
-                "fdadi_hb_set_shape: threads must be a power of two in [64, 1024]");
-  FDADI_REQUIRE((int64_t)(t / kNL) * kMAXL >= n, FDADI_ERR_ARG,
-                "fdadi_hb_set_shape: too few threads for the longest line");
-  if (kind == 0) hb->spot_threads = t; else hb->var_threads = t;
+  const int nl = kind == 0 ? kNLs : kNL;
+  FDADI_REQUIRE(lines == 0 || lines == nl, FDADI_ERR_UNSUPPORTED,
+                "fdadi_hb_set_shape: lines per tile is fixed (4 spot, 8 variance)");
+  FDADI_REQUIRE(threads == 0 || threads == 64 || threads == 128 || threads == 256 ||
+                    threads == 512 || threads == 1024,
+                FDADI_ERR_ARG, "fdadi_hb_set_shape: threads must be a power of two in [64, 1024]");
+  if (kind == 0) {
+    FDADI_RET(spot_shape(hb->nSmax, threads, &hb->spot_threads, &hb->spot_maxl));
+  } else {
+    const int t = threads == 0 ? threads_for(hb->nvmax) : threads;
+    FDADI_REQUIRE((int64_t)(t / kNL) * kMAXL >= hb->nvmax, FDADI_ERR_ARG,
+                  "fdadi_hb_set_shape: too few threads for the longest variance line");
+    hb->var_threads = t;
+  }
   if (hb->gexec) {
     (void)hipGraphExecDestroy(hb->gexec);
     hb->gexec = nullptr;
@@ -1043,7 +1285,7 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
 int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines) {
   FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_get_shape: bad kind");
   if (threads) *threads = kind == 0 ? hb->spot_threads : hb->var_threads;
-  if (lines) *lines = kNL;
+  if (lines) *lines = kind == 0 ? kNLs : kNL;
   return FDADI_OK;
 }
 

@@ -1,0 +1,570 @@
+// Learned price surrogate: an MLP 8 -> 512 -> 512 -> 512 -> 512 -> 1 with
+// SiLU, BF16 weights and activations, FP32 accumulation and FP32 master
+// weights, trained with Adam by hand-written MFMA kernels (gfx950).
+//
+// GEMMs are "NT": C[m,n] = sum_k A[m,k] * B[n,k] with both operands
+// row-major and K-contiguous, bf16.  Every product the network needs is put
+// in that form by keeping transposed copies (written by the producing
+// epilogue, never by a separate transpose):
+//   forward     Z_l   = A_{l-1} W_l^T       A = A_{l-1} [B x in],  B = W_l   [out x in]
+//   backward    dA_l  = dZ_{l+1} W_{l+1}    A = dZ      [B x out], B = W^T   [in x out]
+//   weight grad dW_l  = dZ_l^T A_{l-1}      A = dZ^T    [out x B], B = A^T   [in x B]
+// The input layer is zero-padded from 8 to 128 features so every dimension
+// is a multiple of the 128 x 128 work-group tile.
+#include "common.h"
+
+#include <cmath>
+#include <vector>
+
+typedef __bf16 bf16;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+#define FDADI_RET(x)     \
+  do {                   \
+    int _st = (x);       \
+    if (_st) return _st; \
+  } while (0)
+
+namespace fdadi {
+namespace {
+
+constexpr int kIn = 8;       // Heston features
+constexpr int kInPad = 128;  // padded input width
+constexpr int kH = 512;      // hidden width
+constexpr int kLayers = 4;   // hidden layers (the 512 -> 1 head is separate)
+constexpr int kTile = 128;
+constexpr int kKStep = 32;
+constexpr int kLdsPitch = kKStep + 8;  // bf16 elements per LDS row (16 B pad)
+
+enum Epi { EPI_FWD = 0, EPI_INF = 1, EPI_BWD = 2, EPI_F32 = 3 };
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+struct EpiArgs {
+  const float *bias;  // [N] (EPI_FWD/EPI_INF)
+  float *Z;           // [M x N] pre-activations (EPI_FWD writes, EPI_BWD reads)
+  bf16 *out;          // [M x N] bf16 result
+  bf16 *outT;         // [N x M] bf16 transposed result (may be null)
+  float *C;           // [M x N] fp32 result (EPI_F32)
+};
+
+// C = A B^T on MFMA 16x16x32 bf16.  256 threads = 4 waves in a 2 x 2 grid,
+// each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA blocks.  The K loop stages
+// 128 x 32 tiles of A and B in LDS.
+// MFMA operand layout (v_mfma_f32_16x16x32_bf16): lane l holds
+//   A[i = l%16][k = 8*(l/16) .. +7],  B[k = 8*(l/16) .. +7][j = l%16],
+//   D[i = 4*(l/16) + t][j = l%16] for t = 0..3.
+template <int EPI>
+__global__ void __launch_bounds__(256)
+gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, int N, int K,
+               EpiArgs ep) {
+  __shared__ __attribute__((aligned(16))) bf16 As[kTile * kLdsPitch];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[kTile * kLdsPitch];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int wm = wave / 2, wn = wave % 2;
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // Global -> LDS: each thread moves 2 x 16 B of A and of B per K step.
+  const int lrow = tid / 4, lk = (tid % 4) * 8;
+  for (int k0 = 0; k0 < K; k0 += kKStep) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int r = lrow + 64 * p;
+      const bf16x8 va = *reinterpret_cast<const bf16x8 *>(A + (size_t)(m0 + r) * K + k0 + lk);
+      const bf16x8 vb = *reinterpret_cast<const bf16x8 *>(B + (size_t)(n0 + r) * K + k0 + lk);
+      *reinterpret_cast<bf16x8 *>(As + r * kLdsPitch + lk) = va;
+      *reinterpret_cast<bf16x8 *>(Bs + r * kLdsPitch + lk) = vb;
+    }
+    __syncthreads();
+    bf16x8 af[4], bfr[4];
+    const int kseg = (lane / 16) * 8;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int row = wm * 64 + a * 16 + lane % 16;
+      af[a] = *reinterpret_cast<const bf16x8 *>(As + row * kLdsPitch + kseg);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int col = wn * 64 + b * 16 + lane % 16;
+      bfr[b] = *reinterpret_cast<const bf16x8 *>(Bs + col * kLdsPitch + kseg);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    __syncthreads();
+  }
+  // Epilogue.
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int n = n0 + wn * 64 + b * 16 + lane % 16;
+      const int mb = m0 + wm * 64 + a * 16 + 4 * (lane / 16);
+      float v[4] = {acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
+      if (EPI == EPI_F32) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ep.C[(size_t)(mb + t) * N + n] = v[t];
+        continue;
+      }
+      bf16 o[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const size_t idx = (size_t)(mb + t) * N + n;
+        float r;
+        if (EPI == EPI_BWD) {
+          r = v[t] * dsilu(ep.Z[idx]);
+        } else {
+          const float z = v[t] + ep.bias[n];
+          if (EPI == EPI_FWD) ep.Z[idx] = z;
+          r = silu(z);
+        }
+        o[t] = static_cast<bf16>(r);
+        ep.out[idx] = o[t];
+      }
+      if (ep.outT) {
+        // four consecutive rows of one column -> 8 contiguous bytes of the transpose
+        typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+        bf16x4 w = {o[0], o[1], o[2], o[3]};
+        *reinterpret_cast<bf16x4 *>(ep.outT + (size_t)n * M + mb) = w;
+      }
+    }
+}
+
+// Features -> padded, normalised bf16 input rows (and their transpose):
+// x_f = 2 (p_f - lo_f) / (hi_f - lo_f) - 1 for the 8 Heston parameters.
+__global__ void prep_input_kernel(const double *__restrict__ P, const int *__restrict__ idx,
+                                  int rows, const double *__restrict__ lo,
+                                  const double *__restrict__ hi, bf16 *X, bf16 *XT, int Mpad) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= Mpad) return;
+  const bool valid = r < rows;
+  const int src = valid ? (idx ? idx[r] : r) : 0;
+  for (int f = 0; f < kInPad; ++f) {
+    float v = 0.f;
+    if (valid && f < kIn) v = (float)(2.0 * (P[(size_t)src * kIn + f] - lo[f]) / (hi[f] - lo[f]) - 1.0);
+    const bf16 b = static_cast<bf16>(v);
+    X[(size_t)r * kInPad + f] = b;
+    if (XT) XT[(size_t)f * Mpad + r] = b;
+  }
+}
+
+// Head: y[r] = sum_n A[r,n] w[n] + b  (one wave per row, fp32).
+__global__ void head_forward_kernel(const bf16 *__restrict__ Aact, const bf16 *__restrict__ w,
+                                    const float *__restrict__ b, int rows, float *y) {
+  const int r = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (r >= rows) return;
+  float s = 0.f;
+  for (int n = lane; n < kH; n += 64) s += (float)Aact[(size_t)r * kH + n] * (float)w[n];
+  for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
+  if (lane == 0) y[r] = s + b[0];
+}
+
+// Loss gradient and the head's backward: dy = 2 (y - t) / B (rows >= valid
+// are padding with dy = 0); dZ4 = dy w silu'(Z4) (bf16, and transposed).
+__global__ void head_backward_kernel(const float *__restrict__ y, const double *__restrict__ T,
+                                     const int *__restrict__ idx, int valid, int Mpad,
+                                     const bf16 *__restrict__ w, const float *__restrict__ Z,
+                                     float *dy, bf16 *dZ, bf16 *dZT, float *loss_acc,
+                                     double tmean, double tstd) {
+  const int r = blockIdx.x, n0 = threadIdx.x;
+  float g = 0.f;
+  if (r < valid) {
+    const float t = (float)((T[idx ? idx[r] : r] - tmean) / tstd);
+    const float e = y[r] - t;
+    g = 2.f * e / (float)valid;
+    if (n0 == 0) {
+      dy[r] = g;
+      atomicAdd(loss_acc, e * e / (float)valid);
+    }
+  } else if (n0 == 0) {
+    dy[r] = 0.f;
+  }
+  for (int n = n0; n < kH; n += blockDim.x) {
+    const size_t i = (size_t)r * kH + n;
+    const bf16 v = static_cast<bf16>(g * (float)w[n] * dsilu(Z[i]));
+    dZ[i] = v;
+    dZT[(size_t)n * Mpad + r] = v;
+  }
+}
+
+// Column reduction for the head: gw[n] = sum_r dy[r] A[r,n]; gb = sum_r dy[r].
+__global__ void head_wgrad_kernel(const float *__restrict__ dy, const bf16 *__restrict__ Aact,
+                                  int Mpad, float *gw, float *gb) {
+  const int n = blockIdx.x;  // 0..kH (kH -> bias)
+  float s = 0.f;
+  for (int r = threadIdx.x; r < Mpad; r += blockDim.x)
+    s += dy[r] * (n < kH ? (float)Aact[(size_t)r * kH + n] : 1.f);
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = blockDim.x / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (n < kH) gw[n] = red[0]; else gb[0] = red[0];
+  }
+}
+
+// Bias gradient: row sums of dZ^T [N x Mpad].
+__global__ void bias_grad_kernel(const bf16 *__restrict__ dZT, int Mpad, float *gb) {
+  const int n = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < Mpad; r += blockDim.x) s += (float)dZT[(size_t)n * Mpad + r];
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = blockDim.x / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gb[n] = red[0];
+}
+
+// Adam on the flat fp32 master parameters.
+__global__ void adam_kernel(float *p, const float *__restrict__ g, float *m, float *v, int64_t n,
+                            float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies of a weight matrix W [out x in] and its transpose.
+__global__ void cast_weights_kernel(const float *__restrict__ W, int out, int in, bf16 *Wb,
+                                    bf16 *WbT) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= out * in) return;
+  const int o = i / in, c = i % in;
+  const bf16 b = static_cast<bf16>(W[i]);
+  Wb[i] = b;
+  if (WbT) WbT[(size_t)c * out + o] = b;
+}
+
+__global__ void zero_kernel(float *p, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+__global__ void unscale_kernel(const float *__restrict__ y, int rows, double tmean, double tstd,
+                               double *out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) out[r] = (double)y[r] * tstd + tmean;
+}
+
+}  // namespace
+}  // namespace fdadi
+
+using namespace fdadi;
+
+// Parameter layout in the flat fp32 master array:
+//   W0 [kH x kInPad], b0 [kH], W1..W3 [kH x kH], b1..b3 [kH], w4 [kH], b4 [1].
+struct fdadi_mlp {
+  int max_rows = 0;  // capacity (multiple of 128)
+  int64_t nparam = 0;
+  int64_t off_W[kLayers] = {}, off_b[kLayers] = {}, off_w4 = 0, off_b4 = 0;
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // master, grad, Adam moments
+  bf16 *Wb[kLayers] = {}, *WbT[kLayers] = {}, *w4b = nullptr;
+  bf16 *X = nullptr, *XT = nullptr;
+  bf16 *Act[kLayers] = {}, *ActT[kLayers] = {};
+  float *Z[kLayers] = {};
+  bf16 *dZ = nullptr, *dZT = nullptr, *dZ2 = nullptr, *dZT2 = nullptr;
+  float *y = nullptr, *dy = nullptr, *loss = nullptr;
+  double *lohi = nullptr;  // [lo(8) | hi(8)]
+  double tmean = 0.0, tstd = 1.0;
+  int64_t step = 0;
+};
+
+namespace fdadi {
+namespace {
+
+int in_dim(int l) { return l == 0 ? kInPad : kH; }
+
+int gemm(int epi, const bf16 *A, const bf16 *B, int M, int N, int K, const EpiArgs &ep,
+         hipStream_t s) {
+  FDADI_REQUIRE(M % kTile == 0 && N % kTile == 0 && K % kKStep == 0, FDADI_ERR_ARG,
+                "mlp gemm: dimensions must be multiples of 128 x 128 x 32");
+  dim3 grid(N / kTile, M / kTile);
+  switch (epi) {
+    case EPI_FWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
+    case EPI_INF: hipLaunchKernelGGL(gemm_nt_kernel<EPI_INF>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
+    case EPI_BWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
+    default: hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
+  }
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+int refresh_bf16(fdadi_mlp *m, hipStream_t s) {
+  for (int l = 0; l < kLayers; ++l) {
+    const int n = kH * in_dim(l);
+    hipLaunchKernelGGL(cast_weights_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       m->p + m->off_W[l], kH, in_dim(l), m->Wb[l], m->WbT[l]);
+  }
+  hipLaunchKernelGGL(cast_weights_kernel, dim3(grid_for(kH, 256)), dim3(256), 0, s,
+                     m->p + m->off_w4, 1, kH, m->w4b, (bf16 *)nullptr);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+// Forward over rows [0, Mpad) of the prepared input m->X; train keeps Z and
+// the transposed activations for the backward pass.
+int forward(fdadi_mlp *m, int Mpad, bool train, hipStream_t s) {
+  const bf16 *in = m->X;
+  for (int l = 0; l < kLayers; ++l) {
+    EpiArgs ep{};
+    ep.bias = m->p + m->off_b[l];
+    ep.Z = train ? m->Z[l] : nullptr;
+    ep.out = m->Act[l];
+    ep.outT = train ? m->ActT[l] : nullptr;
+    FDADI_RET(gemm(train ? EPI_FWD : EPI_INF, in, m->Wb[l], Mpad, kH, in_dim(l), ep, s));
+    in = m->Act[l];
+  }
+  hipLaunchKernelGGL(head_forward_kernel, dim3((Mpad + 3) / 4), dim3(256), 0, s,
+                     m->Act[kLayers - 1], m->w4b, m->p + m->off_b4, Mpad, m->y);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+}  // namespace
+}  // namespace fdadi
+
+extern "C" {
+
+int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
+                     const double *lo, const double *hi, double tmean, double tstd,
+                     fdadi_mlp **out) {
+  FDADI_REQUIRE(out && init_params && lo && hi, FDADI_ERR_ARG, "fdadi_mlp_create: null argument");
+  FDADI_REQUIRE(max_rows > 0 && max_rows % kTile == 0, FDADI_ERR_ARG,
+                "fdadi_mlp_create: max_rows must be a positive multiple of 128");
+  fdadi_mlp *m = new fdadi_mlp();
+  m->max_rows = max_rows;
+  int64_t o = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    m->off_W[l] = o;
+    o += (int64_t)kH * in_dim(l);
+    m->off_b[l] = o;
+    o += kH;
+  }
+  m->off_w4 = o;
+  o += kH;
+  m->off_b4 = o;
+  o += 1;
+  m->nparam = o;
+  if (nparam != o) {
+    delete m;
+    set_error("fdadi_mlp_create: expected " + std::to_string(o) + " parameters, got " +
+              std::to_string(nparam));
+    return FDADI_ERR_ARG;
+  }
+  m->tmean = tmean;
+  m->tstd = tstd;
+  hipError_t e = hipSuccess;
+  auto al = [&](void **p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  const size_t R = (size_t)max_rows;
+  al((void **)&m->p, sizeof(float) * o);
+  al((void **)&m->g, sizeof(float) * o);
+  al((void **)&m->m, sizeof(float) * o);
+  al((void **)&m->v, sizeof(float) * o);
+  for (int l = 0; l < kLayers; ++l) {
+    al((void **)&m->Wb[l], sizeof(bf16) * kH * in_dim(l));
+    al((void **)&m->WbT[l], sizeof(bf16) * kH * in_dim(l));
+    al((void **)&m->Act[l], sizeof(bf16) * R * kH);
+    al((void **)&m->ActT[l], sizeof(bf16) * R * kH);
+    al((void **)&m->Z[l], sizeof(float) * R * kH);
+  }
+  al((void **)&m->w4b, sizeof(bf16) * kH);
+  al((void **)&m->X, sizeof(bf16) * R * kInPad);
+  al((void **)&m->XT, sizeof(bf16) * R * kInPad);
+  al((void **)&m->dZ, sizeof(bf16) * R * kH);
+  al((void **)&m->dZT, sizeof(bf16) * R * kH);
+  al((void **)&m->dZ2, sizeof(bf16) * R * kH);
+  al((void **)&m->dZT2, sizeof(bf16) * R * kH);
+  al((void **)&m->y, sizeof(float) * R);
+  al((void **)&m->dy, sizeof(float) * R);
+  al((void **)&m->loss, sizeof(float));
+  al((void **)&m->lohi, sizeof(double) * 2 * kIn);
+  if (e == hipSuccess) e = hipMemcpy(m->p, init_params, sizeof(float) * o, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->lohi, lo, sizeof(double) * kIn, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(m->lohi + kIn, hi, sizeof(double) * kIn, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(o, 256)), dim3(256), 0, 0, m->m, o);
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(o, 256)), dim3(256), 0, 0, m->v, o);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && refresh_bf16(m, 0) != FDADI_OK) e = hipErrorUnknown;
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    set_error(std::string("fdadi_mlp_create: ") + hipGetErrorString(e));
+    // fall through to free what was allocated
+    void *ptrs[] = {m->p, m->g, m->m, m->v, m->w4b, m->X, m->XT, m->dZ, m->dZT, m->dZ2,
+                    m->dZT2, m->y, m->dy, m->loss, m->lohi};
+    for (void *q : ptrs) (void)hipFree(q);
+    for (int l = 0; l < kLayers; ++l) {
+      (void)hipFree(m->Wb[l]); (void)hipFree(m->WbT[l]); (void)hipFree(m->Act[l]);
+      (void)hipFree(m->ActT[l]); (void)hipFree(m->Z[l]);
+    }
+    delete m;
+    return FDADI_ERR_HIP;
+  }
+  *out = m;
+  return FDADI_OK;
+}
+
+int fdadi_mlp_destroy(fdadi_mlp *m) {
+  if (!m) return FDADI_OK;
+  void *ptrs[] = {m->p, m->g, m->m, m->v, m->w4b, m->X, m->XT, m->dZ, m->dZT, m->dZ2,
+                  m->dZT2, m->y, m->dy, m->loss, m->lohi};
+  for (void *q : ptrs) (void)hipFree(q);
+  for (int l = 0; l < kLayers; ++l) {
+    (void)hipFree(m->Wb[l]); (void)hipFree(m->WbT[l]); (void)hipFree(m->Act[l]);
+    (void)hipFree(m->ActT[l]); (void)hipFree(m->Z[l]);
+  }
+  delete m;
+  return FDADI_OK;
+}
+
+int fdadi_mlp_nparam(int64_t *n) {
+  FDADI_REQUIRE(n, FDADI_ERR_ARG, "fdadi_mlp_nparam: null output");
+  *n = (int64_t)kH * kInPad + kH + (kLayers - 1) * ((int64_t)kH * kH + kH) + kH + 1;
+  return FDADI_OK;
+}
+
+int fdadi_mlp_get_params(const fdadi_mlp *m, float *params, float *grads) {
+  FDADI_REQUIRE(m, FDADI_ERR_ARG, "fdadi_mlp_get_params: null model");
+  FDADI_TRY_HIP(hipDeviceSynchronize());
+  if (params) FDADI_TRY_HIP(hipMemcpy(params, m->p, sizeof(float) * m->nparam, hipMemcpyDeviceToHost));
+  if (grads) FDADI_TRY_HIP(hipMemcpy(grads, m->g, sizeof(float) * m->nparam, hipMemcpyDeviceToHost));
+  return FDADI_OK;
+}
+
+int fdadi_mlp_set_params(fdadi_mlp *m, const float *params, void *stream) {
+  FDADI_REQUIRE(m && params, FDADI_ERR_ARG, "fdadi_mlp_set_params: null argument");
+  FDADI_TRY_HIP(hipMemcpy(m->p, params, sizeof(float) * m->nparam, hipMemcpyHostToDevice));
+  FDADI_RET(refresh_bf16(m, as_stream(stream)));
+  return FDADI_OK;
+}
+
+// Gradient of the mean squared error (targets standardised with tmean/tstd)
+// for rows idx[0..rows) of (P [n x 8], T [n]); leaves it in the grad array.
+static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const int *d_idx,
+                     int rows, hipStream_t s) {
+  FDADI_REQUIRE(rows > 0 && rows <= m->max_rows, FDADI_ERR_ARG,
+                "fdadi_mlp: batch rows exceed the model capacity");
+  const int Mpad = (rows + kTile - 1) / kTile * kTile;
+  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s, d_P, d_idx,
+                     rows, m->lohi, m->lohi + kIn, m->X, m->XT, Mpad);
+  FDADI_CHECK_LAUNCH();
+  FDADI_RET(forward(m, Mpad, true, s));
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, m->loss, (int64_t)1);
+  hipLaunchKernelGGL(head_backward_kernel, dim3(Mpad), dim3(256), 0, s, m->y, d_T, d_idx, rows,
+                     Mpad, m->w4b, m->Z[kLayers - 1], m->dy, m->dZ, m->dZT, m->loss, m->tmean,
+                     m->tstd);
+  hipLaunchKernelGGL(head_wgrad_kernel, dim3(kH + 1), dim3(256), 0, s, m->dy,
+                     m->Act[kLayers - 1], Mpad, m->g + m->off_w4, m->g + m->off_b4);
+  FDADI_CHECK_LAUNCH();
+  bf16 *dz = m->dZ, *dzt = m->dZT, *dz2 = m->dZ2, *dzt2 = m->dZT2;
+  for (int l = kLayers - 1; l >= 0; --l) {
+    // dW_l = dZ_l^T A_{l-1};  db_l = column sums of dZ_l
+    const bf16 *aT = l == 0 ? m->XT : m->ActT[l - 1];
+    EpiArgs ew{};
+    ew.C = m->g + m->off_W[l];
+    FDADI_RET(gemm(EPI_F32, dzt, aT, kH, in_dim(l), Mpad, ew, s));
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(kH), dim3(256), 0, s, dzt, Mpad,
+                       m->g + m->off_b[l]);
+    FDADI_CHECK_LAUNCH();
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1})
+      EpiArgs eb{};
+      eb.Z = m->Z[l - 1];
+      eb.out = dz2;
+      eb.outT = dzt2;
+      FDADI_RET(gemm(EPI_BWD, dz, m->WbT[l], Mpad, kH, kH, eb, s));
+      std::swap(dz, dz2);
+      std::swap(dzt, dzt2);
+    }
+  }
+  return FDADI_OK;
+}
+
+int fdadi_mlp_grad(fdadi_mlp *m, const double *d_P, const double *d_T, const int *d_idx,
+                   int32_t rows, float *loss, void *stream) {
+  FDADI_REQUIRE(m && d_P && d_T, FDADI_ERR_ARG, "fdadi_mlp_grad: null argument");
+  hipStream_t s = as_stream(stream);
+  FDADI_RET(grad_impl(m, d_P, d_T, d_idx, rows, s));
+  if (loss) {
+    FDADI_TRY_HIP(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    FDADI_TRY_HIP(hipStreamSynchronize(s));
+  }
+  return FDADI_OK;
+}
+
+int fdadi_mlp_train_step(fdadi_mlp *m, const double *d_P, const double *d_T, const int *d_idx,
+                         int32_t rows, float lr, void *stream) {
+  FDADI_REQUIRE(m && d_P && d_T, FDADI_ERR_ARG, "fdadi_mlp_train_step: null argument");
+  hipStream_t s = as_stream(stream);
+  FDADI_RET(grad_impl(m, d_P, d_T, d_idx, rows, s));
+  m->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+  hipLaunchKernelGGL(adam_kernel, dim3(grid_for(m->nparam, 256)), dim3(256), 0, s, m->p, m->g,
+                     m->m, m->v, m->nparam, lr, b1, b2, 1e-8f, c1, c2);
+  FDADI_CHECK_LAUNCH();
+  FDADI_RET(refresh_bf16(m, s));
+  return FDADI_OK;
+}
+
+int fdadi_mlp_loss(const fdadi_mlp *m, float *loss) {
+  FDADI_REQUIRE(m && loss, FDADI_ERR_ARG, "fdadi_mlp_loss: null argument");
+  FDADI_TRY_HIP(hipDeviceSynchronize());
+  FDADI_TRY_HIP(hipMemcpy(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost));
+  return FDADI_OK;
+}
+
+int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out, void *stream) {
+  FDADI_REQUIRE(m && d_P && d_out && n >= 0, FDADI_ERR_ARG, "fdadi_mlp_predict: bad arguments");
+  hipStream_t s = as_stream(stream);
+  for (int64_t r0 = 0; r0 < n; r0 += m->max_rows) {
+    const int rows = (int)std::min<int64_t>(m->max_rows, n - r0);
+    const int Mpad = (rows + kTile - 1) / kTile * kTile;
+    hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s,
+                       d_P + r0 * kIn, (const int *)nullptr, rows, m->lohi, m->lohi + kIn, m->X,
+                       (bf16 *)nullptr, Mpad);
+    FDADI_CHECK_LAUNCH();
+    FDADI_RET(forward(m, Mpad, false, s));
+    hipLaunchKernelGGL(unscale_kernel, dim3(grid_for(rows, 256)), dim3(256), 0, s, m->y, rows,
+                       m->tmean, m->tstd, d_out + r0);
+    FDADI_CHECK_LAUNCH();
+  }
+  return FDADI_OK;
+}
+
+// Raw NT GEMM for tests: C (fp32, M x N) = A (bf16 bits, M x K) B^T (bf16 bits, N x K).
+int fdadi_mlp_gemm_nt(const uint16_t *d_A, const uint16_t *d_B, int32_t M, int32_t N, int32_t K,
+                      float *d_C, void *stream) {
+  FDADI_REQUIRE(d_A && d_B && d_C, FDADI_ERR_ARG, "fdadi_mlp_gemm_nt: null argument");
+  EpiArgs ep{};
+  ep.C = d_C;
+  return gemm(EPI_F32, reinterpret_cast<const bf16 *>(d_A), reinterpret_cast<const bf16 *>(d_B),
+              M, N, K, ep, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+"""Learned price surrogate with an ADI fallback.
+
+``MLP`` drives the device network of csrc/mlp.hip (BF16 weights and
+activations, FP32 accumulation and master weights, Adam, MFMA GEMMs).
+``Ensemble`` averages several independently initialised networks and gates
+each query on their spread; ``SurrogatePricer`` answers a batch of queries
+with the ensemble mean where the gate passes and the parameters lie inside
+the training box, and with the FP64 batched ADI engine elsewhere.
+
+Features, in order: kappa, theta, sigma, rho, v0, r, T, K/S0 (S0 = 100).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .SizedArrayPtr import SizedArrayPtr
+from .batch import HestonBatch
+from .utils import sinh_space, exponential_space, nearest_index
+
+FEATURES = ("kappa", "theta", "sigma", "rho", "v0", "r", "T", "K/S0")
+BOX_LO = np.array([0.5, 0.01, 0.1, -0.9, 0.01, 0.0, 0.1, 0.7])
+BOX_HI = np.array([5.0, 0.16, 1.0, 0.0, 0.16, 0.08, 3.0, 1.3])
+N_IN, N_PAD, N_H, N_LAYERS = 8, 128, 512, 4
+
+
+def nparam():
+    n = _lib.c_i64()
+    _lib.call("fdadi_mlp_nparam", ctypes.byref(n))
+    return n.value
+
+
+def init_params(seed):
+    """He-normal weights for SiLU layers, zero biases (the padded input
+    columns stay zero and receive zero gradient)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l in range(N_LAYERS):
+        fan_in = N_IN if l == 0 else N_H
+        W = rng.normal(0.0, np.sqrt(2.0 / fan_in), (N_H, N_PAD if l == 0 else N_H))
+        if l == 0:
+            W[:, N_IN:] = 0.0
+        parts += [W.ravel(), np.zeros(N_H)]
+    parts += [rng.normal(0.0, np.sqrt(1.0 / N_H), N_H), np.zeros(1)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == nparam()
+    return p
+
+
+class DeviceInts(object):
+    """int32 device buffer (stored in a double allocation)."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        pad = np.zeros(a.size + (a.size % 2), dtype=np.int32)
+        pad[:a.size] = a
+        self.n = a.size
+        self.buf = SizedArrayPtr(pad.view(np.float64))
+
+    @property
+    def p(self):
+        return self.buf.p
+
+
+class MLP(object):
+
+    def __init__(self, lo=BOX_LO, hi=BOX_HI, tmean=0.0, tstd=1.0, seed=0, max_rows=8192,
+                 params=None):
+        p = init_params(seed) if params is None else np.ascontiguousarray(params, np.float32)
+        self.lo = np.ascontiguousarray(lo, dtype=float)
+        self.hi = np.ascontiguousarray(hi, dtype=float)
+        self.tmean, self.tstd = float(tmean), float(tstd)
+        self.max_rows = int(max_rows)
+        h = ctypes.c_void_p()
+        _lib.call("fdadi_mlp_create", self.max_rows, _lib.ptr(p), p.size, _lib.ptr(self.lo),
+                  _lib.ptr(self.hi), self.tmean, self.tstd, ctypes.byref(h))
+        self._h = h
+        self.stream = _lib.default_stream()
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib.fdadi_mlp_destroy(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def params(self):
+        p = np.zeros(nparam(), dtype=np.float32)
+        _lib.call("fdadi_mlp_get_params", self._h, _lib.ptr(p), None)
+        return p
+
+    def grads(self):
+        g = np.zeros(nparam(), dtype=np.float32)
+        _lib.call("fdadi_mlp_get_params", self._h, None, _lib.ptr(g))
+        return g
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _lib.call("fdadi_mlp_set_params", self._h, _lib.ptr(p), self.stream)
+
+    def grad(self, P, T, idx=None, rows=None):
+        """Loss and gradient on device arrays P [n x 8], T [n] (SizedArrayPtr)."""
+        rows = (idx.n if idx is not None else P.size // N_IN) if rows is None else rows
+        loss = ctypes.c_float()
+        _lib.call("fdadi_mlp_grad", self._h, P.p, T.p, idx.p if idx is not None else None,
+                  int(rows), ctypes.byref(loss), self.stream)
+        return loss.value
+
+    def train_step(self, P, T, idx, rows, lr):
+        _lib.call("fdadi_mlp_train_step", self._h, P.p, T.p, idx.p if idx is not None else None,
+                  int(rows), ctypes.c_float(lr), self.stream)
+
+    def last_loss(self):
+        loss = ctypes.c_float()
+        _lib.call("fdadi_mlp_loss", self._h, ctypes.byref(loss))
+        return loss.value
+
+    def predict_device(self, P, n, out):
+        _lib.call("fdadi_mlp_predict", self._h, P.p, int(n), out.p, self.stream)
+
+    def predict(self, P_host):
+        P_host = np.ascontiguousarray(P_host, dtype=float)
+        n = P_host.shape[0]
+        P = SizedArrayPtr(P_host)
+        out = SizedArrayPtr().alloc(n)
+        self.predict_device(P, n, out)
+        return out.to_numpy()
+
+    def fit(self, P_host, T_host, steps, batch=8192, lr=1e-3, seed=0, log_every=0):
+        """Adam on (P, T) for ``steps`` minibatches drawn without replacement
+        per epoch; returns the loss history (standardised MSE)."""
+        P = SizedArrayPtr(np.ascontiguousarray(P_host, dtype=float))
+        T = SizedArrayPtr(np.ascontiguousarray(T_host, dtype=float))
+        n = T_host.shape[0]
+        rng = np.random.default_rng(seed)
+        perm, pos, hist = rng.permutation(n), 0, []
+        for k in range(steps):
+            if pos + batch > n:
+                perm, pos = rng.permutation(n), 0
+            idx = DeviceInts(perm[pos:pos + batch])
+            pos += batch
+            self.train_step(P, T, idx, min(batch, n), lr)
+            if log_every and (k % log_every == 0 or k == steps - 1):
+                hist.append((k, self.last_loss()))
+        return hist
+
+
+class Ensemble(object):
+    """Independently initialised networks; the spread of their predictions
+    is the per-query error estimate."""
+
+    def __init__(self, members):
+        self.members = list(members)
+
+    def predict(self, P_host):
+        Y = np.stack([m.predict(P_host) for m in self.members])
+        return Y.mean(axis=0), Y.std(axis=0, ddof=1) if len(self.members) > 1 else 0 * Y[0]
+
+
+def label_grids(P, nS=256, nv=128):
+    """Meshes for feature rows P: spot sinh mesh on [0, 5K] clustered at K
+    with S0 = 100 on a node, variance power-law mesh on [0, 5 max(theta, v0)]
+    with v0 on a node (as the batched workload)."""
+    spots, vars_ = [], []
+    for row in P:
+        K = 100.0 * row[7]
+        s = sinh_space(K, 5 * K, 7.0, nS)
+        s[np.argmin(np.abs(s - 100.0))] = 100.0
+        v = exponential_space(0.0, row[4], 5 * max(row[1], row[4]), 4.0, nv)
+        spots.append(s)
+        vars_.append(v)
+    return spots, vars_
+
+
+def adi_prices(P, nS=256, nv=128, nt=200, scheme="cs", chunk=4096, grids=None):
+    """FP64 ADI prices (batched engine) for feature rows P."""
+    P = np.asarray(P, dtype=float)
+    out = np.empty(P.shape[0])
+    for c0 in range(0, P.shape[0], chunk):
+        rows = P[c0:c0 + chunk]
+        if grids is None:
+            spots, vars_ = label_grids(rows, nS, nv)
+        else:
+            spots, vars_ = grids[0][c0:c0 + chunk], grids[1][c0:c0 + chunk]
+        params = [dict(kappa=r[0], theta=r[1], sigma=r[2], rho=r[3], r=r[5]) for r in rows]
+        hb = HestonBatch(params, spots, vars_, [r[6] / nt for r in rows])
+        hb.set_state([np.maximum(s[:, None] - 100.0 * r[7], 0.0) + 0 * v[None, :]
+                      for s, v, r in zip(spots, vars_, rows)])
+        hb.configure(scheme, 0.5, 1.0)
+        hb.step(nt)
+        V = hb.get_state()
+        for k, (s, v, r) in enumerate(zip(spots, vars_, rows)):
+            out[c0 + k] = V[k][nearest_index(s, 100.0), nearest_index(v, r[4])]
+    return out
+
+
+class SurrogatePricer(object):
+    """Ensemble surrogate with a per-query gate and the FP64 ADI fallback."""
+
+    def __init__(self, ensemble, tau, lo=BOX_LO, hi=BOX_HI):
+        self.ensemble, self.tau = ensemble, float(tau)
+        self.lo, self.hi = np.asarray(lo, float), np.asarray(hi, float)
+
+    def gate(self, P, spread):
+        inside = np.all((P >= self.lo) & (P <= self.hi), axis=1)
+        return inside & (spread <= self.tau)
+
+    def price(self, P, **adi_kw):
+        P = np.asarray(P, dtype=float)
+        mean, spread = self.ensemble.predict(P)
+        ok = self.gate(P, spread)
+        out = mean.copy()
+        if (~ok).any():
+            out[~ok] = adi_prices(P[~ok], **adi_kw)
+        return out, ok

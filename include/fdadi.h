@@ -194,6 +194,11 @@ int fdadi_hb_state(fdadi_hb *hb, double **d_state, int64_t *total);
 int fdadi_hb_offsets(const fdadi_hb *hb, int64_t *offsets);
 /* Advance every option by nsteps steps of the configured scheme. */
 int fdadi_hb_step(fdadi_hb *hb, int32_t nsteps, void *stream);
+/* Advance nsteps steps with a HIP event pair around every stage launch and
+ * return, per stage slot of a step (launch order), the summed device time in
+ * milliseconds; *nstages = launches per step.  Synchronises. */
+int fdadi_hb_step_timed(fdadi_hb *hb, int32_t nsteps, void *stream, double *stage_ms,
+                        int32_t max_stages, int32_t *nstages);
 /* Capture nsteps steps into a HIP graph (replaces any previous one) and
  * replay it. */
 int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream);
@@ -204,6 +209,37 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines);
 int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines);
 /* Per-stage launch list for profiling: number of kernel launches per step. */
 int fdadi_hb_launches_per_step(const fdadi_hb *hb, int *n);
+
+/* ---- learned price surrogate ----------------------------------------------------
+ * No counterpart in the reference (its only pricers are the ADI engines and
+ * the COS method, heston.py:490-693).  An MLP 8 -> 512 x4 -> 1 (SiLU) maps
+ * the 8 Heston features (kappa, theta, sigma, rho, v0, r, T, K/S0) to a
+ * price; BF16 weights/activations, FP32 accumulation and master weights,
+ * Adam.  Inputs are normalised with the box [lo, hi]; the target with
+ * (tmean, tstd).  Parameter layout of the flat fp32 array: W0 [512 x 128]
+ * (input zero-padded from 8 to 128 features), b0 [512], W1..W3 [512 x 512],
+ * b1..b3, w4 [512], b4 [1]. */
+typedef struct fdadi_mlp fdadi_mlp;
+int fdadi_mlp_nparam(int64_t *n);
+int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
+                     const double *lo, const double *hi, double tmean, double tstd,
+                     fdadi_mlp **out);
+int fdadi_mlp_destroy(fdadi_mlp *m);
+int fdadi_mlp_get_params(const fdadi_mlp *m, float *params, float *grads);
+int fdadi_mlp_set_params(fdadi_mlp *m, const float *params, void *stream);
+/* Gradient of the mean squared (standardised) error over rows idx[0..rows)
+ * of d_P [n x 8] / d_T [n] (idx NULL = the first rows); *loss if non-NULL. */
+int fdadi_mlp_grad(fdadi_mlp *m, const double *d_P, const double *d_T, const int *d_idx,
+                   int32_t rows, float *loss, void *stream);
+/* grad + one Adam step (beta 0.9/0.999, eps 1e-8) at learning rate lr. */
+int fdadi_mlp_train_step(fdadi_mlp *m, const double *d_P, const double *d_T, const int *d_idx,
+                         int32_t rows, float lr, void *stream);
+int fdadi_mlp_loss(const fdadi_mlp *m, float *loss);
+/* Prices for d_P [n x 8] into d_out [n]. */
+int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out, void *stream);
+/* C (fp32) = A B^T with bf16 operands given as raw bits (test entry point). */
+int fdadi_mlp_gemm_nt(const uint16_t *d_A, const uint16_t *d_B, int32_t M, int32_t N, int32_t K,
+                      float *d_C, void *stream);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,10 @@ of the reference's GPU engine, which is what the product replaces):
   C float, so the CPU engine rounds -r/2 to single precision
   (BandedOperator.pyx:713, FiniteDifferenceEngine.pyx:586).  The GPU engine
   keeps it in double (FiniteDifferenceEngineGPU.pyx:277, 693).
+* ``firsts_reversed`` -- sums the explicit operators in the opposite order.
+  The reference's order is Python 2 dict order (arbitrary); the difference
+  between the two orders is the oracle's own rounding noise floor, which the
+  parity tests use to calibrate the grid tolerance.
 * ``implicit_residual`` -- the CPU implicit scheme keeps the boundary residual
   R in (I - dt A) x = V + dt R (FiniteDifferenceEngine.pyx:631); the GPU
   engine disables it (FiniteDifferenceEngineGPU.pyx:187-188).  Default True
@@ -51,13 +55,14 @@ class HestonOracle(object):
     """
 
     def __init__(self, params, spots, vars, r_split_float32=False,
-                 implicit_residual=True):
+                 implicit_residual=True, firsts_reversed=False):
         self.p = params
         self.spots = np.array(spots, dtype=float)
         self.vars = np.array(vars, dtype=float)
         self.shape = (len(self.spots), len(self.vars))
         self.r_split_float32 = r_split_float32
         self.implicit_residual = implicit_residual
+        self.firsts_reversed = firsts_reversed
         self.t = 0.0
         self.simple_operators = {}
         self.operators = {}
@@ -180,7 +185,13 @@ class HestonOracle(object):
         return [k for k in (0, 1) if k in self.operators]
 
     def _firsts(self, dt):
-        return [self.operators[k] * dt for k in self.operators]
+        # The reference sums the explicit operators in Python 2 dict order
+        # (FiniteDifference/FiniteDifferenceEngine.pyx:958), which is arbitrary;
+        # ``firsts_reversed`` flips the order to measure that rounding noise.
+        keys = list(self.operators)
+        if self.firsts_reversed:
+            keys = keys[::-1]
+        return [self.operators[k] * dt for k in keys]
 
     def _les_lis(self, dt, theta, fold):
         Les = [self.operators[k] * (theta * dt) for k in self._dims()]
