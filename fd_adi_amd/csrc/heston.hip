@@ -490,116 +490,125 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
 }
 
 // ---- variance sweep -------------------------------------------------------------
-// LDS: d[NL * ldv] (line ii, position j at ii*ldv + j) + carries.
-template <int MODE, int THREADS, int NL, int MAXL>
-__global__ void __launch_bounds__(THREADS)
+// Lines along the variance axis are contiguous.  A work-group of 256 threads
+// handles 256/TPL adjacent spot rows of one option; the TPL lanes of a line
+// (a wave or a segment of one) each own a contiguous chunk of L <= MAXL
+// positions in registers, and the chunk carries of both triangular sweeps
+// are combined with a segmented wave scan of affine maps (no LDS, no
+// work-group barrier).  Lines with S != 0 share the 1-D factorisation
+// (VT_M, VT_N, VT_RD, fold_f); the S = 0 line is diagonal.
+template <int MODE, int TPL, int MAXL>
+__global__ void __launch_bounds__(256)
 var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ W,
-                 double *__restrict__ out, int64_t ldv) {
-  extern __shared__ double lds[];
-  constexpr int TPL = THREADS / NL;
-  double *d = lds;
-  double *cM = lds + NL * ldv;
-  double *cE = cM + TPL * NL;
-  double *cIn = cE + TPL * NL;
-  const Tile t = tiles[blockIdx.x];
+                 double *__restrict__ out) {
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const HOpt o = c.opts[t.opt];
-  const int64_t nS = o.nS, nv = o.nv, i0 = t.start;
-  const int tid = threadIdx.x;
+  const int ln = threadIdx.x % TPL;
+  const int64_t i = t.start + threadIdx.x / TPL;
+  const int64_t nS = o.nS, nv = o.nv;
+  const int L = (int)((nv + TPL - 1) / TPL);
+  const int64_t s = (int64_t)ln * L;
+  const int64_t e = s + L < nv ? s + L : nv;
+  const bool row_ok = i < nS;
+  const bool active = row_ok && s < e;
   const double *w = W + o.off;
-
-  // Phase 1: right-hand side (contiguous rows: coalesced).
-  for (int64_t q = tid; q < NL * nv; q += THREADS) {
-    const int64_t ii = q / nv, jq = q % nv, i = i0 + ii;
+  const bool diag = row_ok && SP(c, o, SP_S, i) == 0.0;
+  const bool fact = active && !diag;
+  double d[MAXL], mreg[MAXL], nreg[MAXL], rreg[MAXL];
+  double dl = 0.0;  // this lane's last value (for the bottom fold)
+#pragma unroll
+  for (int k = 0; k < MAXL; ++k) {
+    const int64_t j = s + k;
     double val = 0.0;
-    if (i < nS) {
+    if (active && j < e) {
       if (MODE == VM_IMPL) {
         auto xw = [&](int64_t a, int64_t b) { return w[a * nv + b]; };
-        val = w[i * nv + jq] + o.dt * a01_dot(c, o, i, jq, xw);
+        val = w[i * nv + j] + o.dt * a01_dot(c, o, i, j, xw);
       } else {
-        val = w[i * nv + jq];
+        val = w[i * nv + j];
       }
     }
-    d[ii * ldv + jq] = val;
+    d[k] = val;
+    if (active && j == e - 1) dl = val;
   }
-  __syncthreads();
-
-  // Phase 2: lines with S != 0 share the 1-D factorisation; S = 0 is diagonal.
-  const int ii = tid % NL, ch = tid / NL;
-  const int64_t i = i0 + ii;
-  const int L = (int)((nv + TPL - 1) / TPL);
-  const int64_t s = (int64_t)ch * L;
-  const int64_t e = s + L < nv ? s + L : nv;
-  const bool active = i < nS && s < e;
-  const bool diag = active && SP(c, o, SP_S, i) == 0.0;
-  double *dl = d + ii * ldv;
+  // Bottom fold of the right-hand side: d[nv-1] += f d[nv-2].
+  const double prev_last = __shfl_up(dl, 1, TPL);
+  if (fact && e == nv) {
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k)
+      if (s + k == nv - 1) d[k] += o.fold_f * (k > 0 ? d[k > 0 ? k - 1 : 0] : prev_last);
+  }
+  // Local forward recurrence y = d + m y.
   double M = 1.0, E = 0.0;
-  // Bottom fold of the RHS: d[nv-1] += f d[nv-2] (rows are only read here).
-  if (active && !diag && e == nv) dl[nv - 1] += o.fold_f * dl[nv - 2];
-  __syncthreads();
-  if (active && !diag) {
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k) {
-      const int64_t jq = s + k;
-      if (jq < e) {
-        const double m = VT(c, o, VT_M, jq);
-        E = dl[jq] + m * E;
-        M = m * M;
-      }
+  for (int k = 0; k < MAXL; ++k) {
+    const int64_t j = s + k;
+    if (fact && j < e) {
+      const double m = VT(c, o, VT_M, j);
+      mreg[k] = m;
+      nreg[k] = VT(c, o, VT_N, j);
+      rreg[k] = VT(c, o, VT_RD, j);
+      E = d[k] + m * E;
+      M = m * M;
     }
   }
-  cM[ch * NL + ii] = (active && !diag) ? M : 1.0;
-  cE[ch * NL + ii] = (active && !diag) ? E : 0.0;
-  __syncthreads();
-  affine_scan<THREADS, NL, false>(cM, cE, cIn, TPL);
-  __syncthreads();
-  if (active && !diag) {
-    double y = cIn[ch * NL + ii];
+  // Segmented inclusive prefix scan, then the carry entering each chunk.
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k) {
-      const int64_t jq = s + k;
-      if (jq < e) {
-        y = dl[jq] + VT(c, o, VT_M, jq) * y;
-        dl[jq] = y;
-      }
-    }
-    M = 1.0;
-    E = 0.0;
-#pragma unroll
-    for (int k = MAXL - 1; k >= 0; --k) {
-      const int64_t jq = s + k;
-      if (jq < e) {
-        const double n = VT(c, o, VT_N, jq);
-        E = VT(c, o, VT_RD, jq) * dl[jq] + n * E;
-        M = n * M;
-      }
+  for (int dd = 1; dd < TPL; dd <<= 1) {
+    const double Mo = __shfl_up(M, dd, TPL), Eo = __shfl_up(E, dd, TPL);
+    if (ln >= dd) {
+      E = E + M * Eo;
+      M = M * Mo;
     }
   }
-  __syncthreads();
-  cM[ch * NL + ii] = (active && !diag) ? M : 1.0;
-  cE[ch * NL + ii] = (active && !diag) ? E : 0.0;
-  __syncthreads();
-  affine_scan<THREADS, NL, true>(cM, cE, cIn, TPL);
-  __syncthreads();
-  if (active && !diag) {
-    double x = cIn[ch * NL + ii];
+  double carry = __shfl_up(E, 1, TPL);
+  if (ln == 0) carry = 0.0;
+  M = 1.0;
+  E = 0.0;
+  if (fact) {
+    double y = carry;
 #pragma unroll
-    for (int k = MAXL - 1; k >= 0; --k) {
-      const int64_t jq = s + k;
-      if (jq < e) {
-        x = VT(c, o, VT_RD, jq) * dl[jq] + VT(c, o, VT_N, jq) * x;
-        dl[jq] = x;
+    for (int k = 0; k < MAXL; ++k)
+      if (s + k < e) {
+        y = d[k] + mreg[k] * y;
+        d[k] = y;
       }
-    }
-  } else if (diag) {
-    for (int64_t jq = s; jq < e; ++jq) dl[jq] *= o.rdS0;
+    // Local backward recurrence x = rd y + n x'.
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k)
+      if (s + k < e) {
+        E = rreg[k] * d[k] + nreg[k] * E;
+        M = nreg[k] * M;
+      }
   }
-  __syncthreads();
-
-  // Phase 3: write back.
-  double *ob = out + o.off;
-  for (int64_t q = tid; q < NL * nv; q += THREADS) {
-    const int64_t iq = q / nv, jq = q % nv;
-    if (i0 + iq < nS) ob[(i0 + iq) * nv + jq] = d[iq * ldv + jq];
+  // Segmented inclusive suffix scan (maps of higher lanes applied first).
+#pragma unroll
+  for (int dd = 1; dd < TPL; dd <<= 1) {
+    const double Mo = __shfl_down(M, dd, TPL), Eo = __shfl_down(E, dd, TPL);
+    if (ln + dd < TPL) {
+      E = E + M * Eo;
+      M = M * Mo;
+    }
+  }
+  double after = __shfl_down(E, 1, TPL);
+  if (ln == TPL - 1) after = 0.0;
+  double *ob = out + o.off + i * nv;
+  if (fact) {
+    double x = after;
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k)
+      if (s + k < e) {
+        x = rreg[k] * d[k] + nreg[k] * x;
+        d[k] = x;
+      }
+  } else if (active) {
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) d[k] *= o.rdS0;
+  }
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k)
+      if (s + k < e) ob[s + k] = d[k];
   }
 }
 
@@ -796,7 +805,9 @@ struct fdadi_hb {
   int64_t *d_sp_row = nullptr, *d_vt_row = nullptr;
   Tile *d_stiles = nullptr, *d_vtiles = nullptr, *d_ftiles = nullptr;
   int nstiles = 0, nvtiles = 0, nftiles = 0;
-  int spot_threads = 0, var_threads = 0;
+  int spot_threads = 0;
+  int var_tpl = 1, var_rows = 256;  // variance sweep: lanes per line, rows per 256-thread tile
+  std::vector<int> order;           // options, largest first
   int spot_maxl = 8;  // rows per thread in the spot sweep (8, or 16 for long lines)
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
@@ -870,6 +881,29 @@ int allow_lds(const void *fn, size_t bytes) {
   return FDADI_OK;
 }
 
+// Lanes per variance line so that each lane's chunk fits kMAXL registers.
+int var_tpl_for(int64_t nv) {
+  int t = 1;
+  while ((int64_t)t * kMAXL < nv && t < 64) t *= 2;
+  return t;
+}
+
+// (Re)build the variance-sweep tile list: var_rows spot rows per tile.
+int build_var_tiles(fdadi_hb *hb) {
+  std::vector<Tile> vtl;
+  for (int k : hb->order) {
+    const HOpt &o = hb->h_opts[k];
+    for (int64_t i0 = 0; i0 < o.nS; i0 += hb->var_rows) vtl.push_back(Tile{k, (int32_t)i0});
+  }
+  if (hb->d_vtiles) FDADI_TRY_HIP(hipFree(hb->d_vtiles));
+  hb->d_vtiles = nullptr;
+  FDADI_TRY_HIP(hipMalloc((void **)&hb->d_vtiles, sizeof(Tile) * vtl.size()));
+  FDADI_TRY_HIP(hipMemcpy(hb->d_vtiles, vtl.data(), sizeof(Tile) * vtl.size(),
+                          hipMemcpyHostToDevice));
+  hb->nvtiles = (int)vtl.size();
+  return FDADI_OK;
+}
+
 int threads_for(int64_t n, int nl = kNL, int maxl = kMAXL) {
   // Threads per workgroup so that each line's chunk fits MAXL registers.
   const int64_t need = (n + maxl - 1) / maxl * nl;
@@ -912,28 +946,26 @@ int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *ou
   return FDADI_ERR_UNSUPPORTED;
 }
 
-template <int MODE, int T>
+template <int MODE, int TPL>
 int launch_var_t(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
-  constexpr int TPL = T / kNL;
-  const int64_t ldv = hb->nvmax + 1;
-  const size_t lds = sizeof(double) * (kNL * ldv + 3 * TPL * kNL);
-  FDADI_RET(allow_lds((const void *)var_sweep_kernel<MODE, T, kNL, kMAXL>, lds));
-  hipLaunchKernelGGL((var_sweep_kernel<MODE, T, kNL, kMAXL>), dim3(hb->nvtiles), dim3(T), lds, s,
-                     ctx_of(hb), hb->d_vtiles, W, out, ldv);
+  hipLaunchKernelGGL((var_sweep_kernel<MODE, TPL, kMAXL>), dim3(hb->nvtiles), dim3(256), 0, s,
+                     ctx_of(hb), hb->d_vtiles, W, out);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
 
 template <int MODE>
 int launch_var(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
-  switch (hb->var_threads) {
+  switch (hb->var_tpl) {
+    case 1: return launch_var_t<MODE, 1>(hb, W, out, s);
+    case 2: return launch_var_t<MODE, 2>(hb, W, out, s);
+    case 4: return launch_var_t<MODE, 4>(hb, W, out, s);
+    case 8: return launch_var_t<MODE, 8>(hb, W, out, s);
+    case 16: return launch_var_t<MODE, 16>(hb, W, out, s);
+    case 32: return launch_var_t<MODE, 32>(hb, W, out, s);
     case 64: return launch_var_t<MODE, 64>(hb, W, out, s);
-    case 128: return launch_var_t<MODE, 128>(hb, W, out, s);
-    case 256: return launch_var_t<MODE, 256>(hb, W, out, s);
-    case 512: return launch_var_t<MODE, 512>(hb, W, out, s);
-    case 1024: return launch_var_t<MODE, 1024>(hb, W, out, s);
   }
-  set_error("variance sweep: unsupported thread count");
+  set_error("variance sweep: unsupported lanes per line");
   return FDADI_ERR_UNSUPPORTED;
 }
 
@@ -1070,14 +1102,16 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   for (int k : order) {
     const HOpt &o = hb->h_opts[k];
     for (int64_t j0 = 0; j0 < o.nv; j0 += kNLs) st.push_back(Tile{k, (int32_t)j0});
-    for (int64_t i0 = 0; i0 < o.nS; i0 += kNL) vtl.push_back(Tile{k, (int32_t)i0});
+
     for (int64_t j0 = 0; j0 < o.nv; j0 += 64) ft.push_back(Tile{k, (int32_t)j0});
   }
   hb->nstiles = (int)st.size();
-  hb->nvtiles = (int)vtl.size();
+
   hb->nftiles = (int)ft.size();
   spot_shape(hb->nSmax, 0, &hb->spot_threads, &hb->spot_maxl);
-  hb->var_threads = threads_for(hb->nvmax);
+  hb->order = order;
+  hb->var_tpl = var_tpl_for(hb->nvmax);
+  hb->var_rows = 256 / hb->var_tpl;
   hipError_t e = hipSuccess;
   auto dalloc = [&](void **p, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 8);
@@ -1095,7 +1129,6 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   dalloc((void **)&hb->d_sp_row, sizeof(int64_t) * so);
   dalloc((void **)&hb->d_vt_row, sizeof(int64_t) * vo);
   dalloc((void **)&hb->d_stiles, sizeof(Tile) * st.size());
-  dalloc((void **)&hb->d_vtiles, sizeof(Tile) * vtl.size());
   dalloc((void **)&hb->d_ftiles, sizeof(Tile) * ft.size());
   auto up = [&](void *dst, const void *src, size_t bytes) {
     if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -1106,8 +1139,8 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   up(hb->d_sp_row, sp_row.data(), sizeof(int64_t) * so);
   up(hb->d_vt_row, vt_row.data(), sizeof(int64_t) * vo);
   up(hb->d_stiles, st.data(), sizeof(Tile) * st.size());
-  up(hb->d_vtiles, vtl.data(), sizeof(Tile) * vtl.size());
   up(hb->d_ftiles, ft.data(), sizeof(Tile) * ft.size());
+  if (e == hipSuccess && build_var_tiles(hb) != FDADI_OK) e = hipErrorUnknown;
   if (e == hipSuccess) {
     hipLaunchKernelGGL(spot_tables_kernel, dim3(grid_for(so, 256)), dim3(256), 0, s,
                        hb->d_opts, nopt, hb->d_sp_row, hb->d_spots, hb->d_sp, so, hb->d_spr);
@@ -1257,19 +1290,24 @@ int fdadi_hb_replay(fdadi_hb *hb, void *stream) {
 
 int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
   FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_set_shape: bad kind");
-  const int nl = kind == 0 ? kNLs : kNL;
-  FDADI_REQUIRE(lines == 0 || lines == nl, FDADI_ERR_UNSUPPORTED,
-                "fdadi_hb_set_shape: lines per tile is fixed (4 spot, 8 variance)");
-  FDADI_REQUIRE(threads == 0 || threads == 64 || threads == 128 || threads == 256 ||
-                    threads == 512 || threads == 1024,
-                FDADI_ERR_ARG, "fdadi_hb_set_shape: threads must be a power of two in [64, 1024]");
+  FDADI_REQUIRE(lines == 0, FDADI_ERR_UNSUPPORTED,
+                "fdadi_hb_set_shape: lines per tile follow from the shape (pass 0)");
   if (kind == 0) {
+    FDADI_REQUIRE(threads == 0 || threads == 64 || threads == 128 || threads == 256 ||
+                      threads == 512 || threads == 1024,
+                  FDADI_ERR_ARG,
+                  "fdadi_hb_set_shape: threads must be a power of two in [64, 1024]");
     FDADI_RET(spot_shape(hb->nSmax, threads, &hb->spot_threads, &hb->spot_maxl));
   } else {
-    const int t = threads == 0 ? threads_for(hb->nvmax) : threads;
-    FDADI_REQUIRE((int64_t)(t / kNL) * kMAXL >= hb->nvmax, FDADI_ERR_ARG,
-                  "fdadi_hb_set_shape: too few threads for the longest variance line");
-    hb->var_threads = t;
+    // kind 1: `threads` = lanes per variance line (power of two <= 64)
+    const int t = threads == 0 ? var_tpl_for(hb->nvmax) : threads;
+    FDADI_REQUIRE(t >= 1 && t <= 64 && (t & (t - 1)) == 0, FDADI_ERR_ARG,
+                  "fdadi_hb_set_shape: lanes per variance line must be a power of two <= 64");
+    FDADI_REQUIRE((int64_t)t * kMAXL >= hb->nvmax, FDADI_ERR_ARG,
+                  "fdadi_hb_set_shape: too few lanes for the longest variance line");
+    hb->var_tpl = t;
+    hb->var_rows = 256 / t;
+    FDADI_RET(build_var_tiles(hb));
   }
   if (hb->gexec) {
     (void)hipGraphExecDestroy(hb->gexec);
@@ -1284,8 +1322,8 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
 
 int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines) {
   FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_get_shape: bad kind");
-  if (threads) *threads = kind == 0 ? hb->spot_threads : hb->var_threads;
-  if (lines) *lines = kind == 0 ? kNLs : kNL;
+  if (threads) *threads = kind == 0 ? hb->spot_threads : hb->var_tpl;
+  if (lines) *lines = kind == 0 ? kNLs : hb->var_rows;
   return FDADI_OK;
 }
 

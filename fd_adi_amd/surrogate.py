@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from .SizedArrayPtr import SizedArrayPtr
 from .batch import HestonBatch
-from .utils import sinh_space, exponential_space, nearest_index
+from .utils import sinh_space_batch, exponential_space_batch, nearest_index
 
 FEATURES = ("kappa", "theta", "sigma", "rho", "v0", "r", "T", "K/S0")
 BOX_LO = np.array([0.5, 0.01, 0.1, -0.9, 0.01, 0.0, 0.1, 0.7])
@@ -160,18 +160,21 @@ class Ensemble(object):
 
 
 def label_grids(P, nS=256, nv=128):
-    """Meshes for feature rows P: spot sinh mesh on [0, 5K] clustered at K
-    with S0 = 100 on a node, variance power-law mesh on [0, 5 max(theta, v0)]
-    with v0 on a node (as the batched workload)."""
-    spots, vars_ = [], []
-    for row in P:
-        K = 100.0 * row[7]
-        s = sinh_space(K, 5 * K, 7.0, nS)
-        s[np.argmin(np.abs(s - 100.0))] = 100.0
-        v = exponential_space(0.0, row[4], 5 * max(row[1], row[4]), 4.0, nv)
-        spots.append(s)
-        vars_.append(v)
-    return spots, vars_
+    """Meshes for feature rows P (vectorised): spot sinh mesh on [0, 5K]
+    clustered at K with K and S0 = 100 on nodes, variance power-law mesh on
+    [0, 5 max(theta, v0)] with v0 on a node."""
+    P = np.asarray(P, dtype=float)
+    K = 100.0 * P[:, 7]
+    S = sinh_space_batch(K, 5 * K, 7.0, nS)
+    rows = np.arange(S.shape[0])
+    j = np.argmin(np.abs(S - 100.0), axis=1)
+    # where K itself is the node nearest 100, put S0 on the neighbour on the
+    # other side of 100 (monotonicity is kept: 100 lies between the two)
+    clash = S[rows, j] == K
+    j = np.where(clash, np.where(K < 100.0, j + 1, j - 1), j)
+    S[rows, j] = 100.0
+    V = exponential_space_batch(0.0, P[:, 4], 5 * np.maximum(P[:, 1], P[:, 4]), 4.0, nv)
+    return list(S), list(V)
 
 
 def adi_prices(P, nS=256, nv=128, nt=200, scheme="cs", chunk=4096, grids=None):

@@ -88,3 +88,42 @@ def nearest_index(mesh, value, decimals=4):
     (bisect on the mesh rounded to ``decimals``,
     FiniteDifferenceEngineGPU.pyx:727-734)."""
     return bisect_left(list(np.round(mesh, decimals)), round(float(value), decimals))
+
+
+def sinh_space_batch(exact, high, density, size):
+    """Sinh meshes for many (exact, high) rows at once: the mesh family of
+    ``sinh_space`` on [0, high] with the node nearest ``exact`` forced onto it,
+    without the retuning of the top end (which needs a scalar bounded
+    minimisation per mesh).  Used for the surrogate's label grids, where the
+    same function builds the grids of the training labels and of the ADI
+    fallback."""
+    exact = np.asarray(exact, dtype=float)
+    high = np.asarray(high, dtype=float) * np.ones_like(exact)
+    density = float(density)
+    k = np.arange(size)
+    deps = 1.0 / size * (np.arcsinh((high - exact) / density) - np.arcsinh(-exact / density))
+    eps = np.arcsinh(-exact / density)[:, None] + k[None, :] * deps[:, None]
+    sp = exact[:, None] + density * np.sinh(eps)
+    sp -= sp.min(axis=1, keepdims=True)
+    rows = np.arange(sp.shape[0])
+    sp[rows, np.argmin(np.abs(sp - exact[:, None]), axis=1)] = exact
+    return sp
+
+
+def exponential_space_batch(low, exact, high, ex, size):
+    """Vectorised exponential_space (force_exact) for many (exact, high) rows."""
+    exact = np.asarray(exact, dtype=float)
+    high = np.asarray(high, dtype=float) * np.ones_like(exact)
+    lo = pow(low, 1.0 / ex)
+    hi = high ** (1.0 / ex)
+    x = exact ** (1.0 / ex)
+    dv = (hi - lo) / (size - 1)
+    i = np.arange(size)
+    above = (lo + i[None, :] * dv[:, None]) > x[:, None]
+    j = np.argmax(above, axis=1) - 1
+    if np.any(j <= 0) or not np.all(above.any(axis=1)):
+        raise AssertionError("Did not find suitable value.")
+    dx = x - (lo + j * dv)
+    hi = hi + (size - 1) * dx / j
+    dv = (hi - lo) / (size - 1)
+    return lo + (i[None, :] * dv[:, None]) ** ex

@@ -203,8 +203,11 @@ int fdadi_hb_step_timed(fdadi_hb *hb, int32_t nsteps, void *stream, double *stag
  * replay it. */
 int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream);
 int fdadi_hb_replay(fdadi_hb *hb, void *stream);
-/* Launch shape per sweep kind (0 spot, 1 var): threads per workgroup and
- * lines per tile; used by the autotuner.  0 = library default. */
+/* Launch shape per sweep kind, used by the autotuner (0 = library default):
+ * kind 0 (spot sweep): `threads` per work-group (64..1024, 4 columns per tile);
+ * kind 1 (variance sweep): `threads` = lanes per variance line (1..64, the
+ * work-group has 256 threads).  `lines` must be 0; get_shape reports the
+ * columns / rows per tile. */
 int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines);
 int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines);
 /* Per-stage launch list for profiling: number of kernel launches per step. */

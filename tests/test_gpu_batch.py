@@ -83,11 +83,11 @@ def test_graph_replay_equals_eager(fd):
 def test_launch_shapes_agree(fd):
     P, S, Vs, dts, O = draw(3, 3, nS_range=(40, 100), nv_range=(20, 60))
     outs = []
-    for t in (0, 256, 512):
+    for ts, tv in ((0, 0), (256, 16), (512, 64)):
         hb = fd.HestonBatch(P, S, Vs, dts)
-        if t:
-            hb.set_shape(0, t)
-            hb.set_shape(1, t)
+        if ts:
+            hb.set_shape(0, ts)
+            hb.set_shape(1, tv)
         hb.set_state([o.payoff() for o in O])
         hb.configure("hv", 0.5)
         hb.step(10)
