@@ -309,156 +309,218 @@ __device__ __forceinline__ int xcd_tile(int k, int G) {
   return x * q + (x < rem ? x : rem) + y;
 }
 
-// A spot sweep: one work-group per (option, NL adjacent variance columns);
-// thread (jj, ch) owns rows [ch*L, ch*L+L) of column j0+jj through every
-// phase.  LDS: d[nSmax*NL] (right-hand side, then the solution),
-// e[nSmax*NL] (explicit term of the next stage), 3*TPL*NL carries.
+// Broadcast lane Q of each group of four lanes (DPP quad_perm [Q,Q,Q,Q]).
+template <int Q>
+__device__ __forceinline__ double quad_bcast(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), Q * 0x55, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), Q * 0x55, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Spot-table row shared by a quad of lanes: lane jj holds doubles [4jj, 4jj+4)
+// of the 128-byte row (two 16-byte loads, so a quad reads each row once) and
+// the fields are broadcast.  Every lane of the quad must execute this.
+__device__ __forceinline__ void quad_row_load(const double *__restrict__ spr, int jj, double2 &a,
+                                              double2 &b) {
+  const double2 *p = reinterpret_cast<const double2 *>(spr + 4 * jj);
+  a = p[0];
+  b = p[1];
+}
+
+__device__ __forceinline__ SpotRow quad_row_fields(double2 a, double2 b) {
+  SpotRow r;
+  r.psub = quad_bcast<0>(a.x); r.pmid = quad_bcast<0>(a.y);
+  r.psup = quad_bcast<0>(b.x); r.qsub = quad_bcast<0>(b.y);
+  r.qmid = quad_bcast<1>(a.x); r.qsup = quad_bcast<1>(a.y);
+  r.pr = quad_bcast<1>(b.x);   r.qr = quad_bcast<1>(b.y);
+  r.wsub = quad_bcast<2>(a.x); r.wmid = quad_bcast<2>(a.y);
+  r.wsup = quad_bcast<2>(b.x); r.S = quad_bcast<2>(b.y);
+  r.dir = quad_bcast<3>(a.x);
+  return r;
+}
+
+// Row r (clamped to the grid) of a field at columns jm, j, jp; 32-bit offsets
+// (an option's grid has at most 2048 x 1024 nodes).
+__device__ __forceinline__ void load_row32(const double *__restrict__ f, int r, int nS, int nv,
+                                           int jm, int j, int jp, double out[3]) {
+  const int rr = r < 0 ? 0 : (r >= nS ? nS - 1 : r);
+  const double *p = f + rr * nv;
+  out[0] = p[jm];
+  out[1] = p[j];
+  out[2] = p[jp];
+}
+
+// A spot sweep: one work-group per (option, NL = 4 adjacent variance
+// columns); thread (jj, ch) = (tid % 4, tid / 4) owns rows [ch*L, ch*L+L) of
+// column j0+jj through every phase.  The four lanes of a quad work on the
+// same rows, so they share the spot-table rows.  The row walk keeps its loads
+// two rows ahead of the arithmetic (table rows, pivots, and the state rows
+// entering the 3x3 windows); a scheduling barrier per row stops the compiler
+// from hoisting more than that, which would exhaust the registers.  Loads use
+// clamped indices; rows past the chunk end are computed and discarded.
+//
+// The LU solve is run in scaled form: with pivots rd = 1/u, multipliers
+// mu_i = -a_i rd_i and n_i = -c_i rd_i, the forward sweep computes
+// y'_i = rd_i d_i + mu_i y'_{i-1} (= rd_i y_i) and the backward sweep
+// x_i = y'_i + n_i x_{i+1}, so the pivots are needed only while the
+// right-hand side is formed.  Registers: mu and n of the chunk.  LDS, in
+// thread-private slots [k*THREADS + tid]: y' (then x) and the next stage's
+// explicit term; plus the 3*TPL*NL chunk carries of the scans.
 template <int MODE, int THREADS, int NL, int MAXL>
 __global__ void __launch_bounds__(THREADS)
 spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ U,
-                  const double *__restrict__ Y, double *__restrict__ out, int64_t nSmax) {
+                  const double *__restrict__ Y, double *__restrict__ out) {
+  static_assert(NL == 4, "spot tiles are one quad of columns wide");
   extern __shared__ double lds[];
   constexpr int TPL = THREADS / NL;
-  double *d = lds;
-  double *eb = lds + nSmax * NL;
-  double *cM = eb + nSmax * NL;
+  constexpr bool HAS_Y = MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS;
+  double *db = lds;
+  double *eb = db + MAXL * THREADS;
+  double *cM = eb + MAXL * THREADS;
   double *cE = cM + TPL * NL;
   double *cIn = cE + TPL * NL;
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const HOpt o = c.opts[t.opt];
-  const int64_t nS = o.nS, nv = o.nv, j0 = t.start;
+  const int nS = (int)o.nS, nv = (int)o.nv, j0 = t.start;
   const int tid = threadIdx.x;
   const int jj = tid % NL, ch = tid / NL;
-  const int64_t j = j0 + jj;
-  const int L = (int)((nS + TPL - 1) / TPL);
-  const int64_t s = (int64_t)ch * L;
-  const int64_t e = s + L < nS ? s + L : nS;
-  const bool active = j < nv && s < e;
+  const int j = j0 + jj;
+  const bool col_ok = j < nv;
+  const int jc = col_ok ? j : nv - 1;  // clamped column for loads
+  const int L = (nS + TPL - 1) / TPL;
+  const int s = ch * L;
+  const int e = s + L < nS ? s + L : nS;
+  const bool active = col_ok && s < e;
   const double *u = U + o.off;
-  const double *y2 = (MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS) ? Y + o.off : nullptr;
+  const double *y2 = HAS_Y ? Y + o.off : nullptr;
   const double *rd = c.rd0 + o.off;
   const double *spr = c.spr + o.so * kSpRow;
-  double mreg[MAXL], rdreg[MAXL];
-  double M = 1.0, E = 0.0;
+  auto clamp_row = [&](int i) { return i < 0 ? 0 : (i >= nS ? nS - 1 : i); };
 
-  // Phase 1: walk the chunk with a 3x3 window, form the fused right-hand side
+  VarCol q;
+  q.gsub = VT(c, o, VT_GSUB, jc); q.gmid = VT(c, o, VT_GMID, jc); q.gsup = VT(c, o, VT_GSUP, jc);
+  q.wv0 = VT(c, o, VT_WSUB, jc); q.wv1 = VT(c, o, VT_WMID, jc); q.wv2 = VT(c, o, VT_WSUP, jc);
+  q.v = VT(c, o, VT_V, jc);
+  const int jm = jc > 0 ? jc - 1 : 0, jp = jc < nv - 1 ? jc + 1 : nv - 1;
+  const int jm2 = jc > 1 ? jc - 2 : 0;
+  const bool last = jc == nv - 1;
+  const double tdt = o.tdt, dt = o.dt;
+
+  // Phase 1: walk the chunk with 3x3 windows, form the fused right-hand side
   // (and the next stage's explicit term), and run the local forward recurrence.
-  if (active) {
-    VarCol q;
-    q.gsub = VT(c, o, VT_GSUB, j); q.gmid = VT(c, o, VT_GMID, j); q.gsup = VT(c, o, VT_GSUP, j);
-    q.wv0 = VT(c, o, VT_WSUB, j); q.wv1 = VT(c, o, VT_WMID, j); q.wv2 = VT(c, o, VT_WSUP, j);
-    q.v = VT(c, o, VT_V, j);
-    const int64_t jm = j > 0 ? j - 1 : 0, jp = j < nv - 1 ? j + 1 : nv - 1;
-    const bool last = j == nv - 1;
-    const double tdt = o.tdt, dt = o.dt;
-    Win wu, wy;
-    load_row(u, s - 1, nS, nv, jm, j, jp, wu.v[0]);
-    load_row(u, s, nS, nv, jm, j, jp, wu.v[1]);
-    load_row(u, s + 1, nS, nv, jm, j, jp, wu.v[2]);
-    if (y2) {
-      load_row(y2, s - 1, nS, nv, jm, j, jp, wy.v[0]);
-      load_row(y2, s, nS, nv, jm, j, jp, wy.v[1]);
-      load_row(y2, s + 1, nS, nv, jm, j, jp, wy.v[2]);
-    }
-    double rdprev = s > 0 ? rd[(s - 1) * nv + j] : 0.0;
+  double mu[MAXL], nreg[MAXL];
+  double wu[4][3], wy[4][3];  // rows i-1, i, i+1 and i+2 (in flight)
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k) {
-      const int64_t i = s + k;
-      if (i < e) {
-        const SpotRow R = load_spot_row(spr + i * kSpRow);
-        const bool dm = i == 1, dp = o.barrier && i + 1 == nS - 1;
-        const double uj2 = last ? u[i * nv + j - 2] : 0.0;
-        double rhs = 0.0, ex = 0.0;
-        if (MODE == SM_IMPL) {
-          rhs = wu.v[1][1] + dt * (R.pr + q.v * R.qr);
-        } else {
-          const double a0u = a0w(R, q.v, wu, dm, dp);
-          const double a1u = a1w(o, R, q, wu, last, uj2);
-          const double r0 = R.pr + q.v * R.qr;  // zero on Dirichlet rows
-          const double fu = dt * (a0u + r0) + dt * a1u + dt * a01w(o, R, q, wu);
-          rhs = wu.v[1][1] + fu;
-          if (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) ex = tdt * a1u;
-          if (MODE == SM_FIRST) {
-            rhs -= tdt * a0u;
-          } else {
-            // Operators are linear: A (Y2 - U) = A Y2 - A U.
-            const double yj2 = last ? y2[i * nv + j - 2] : 0.0;
-            const double a0y = a0w(R, q.v, wy, dm, dp);
-            const double a1y = a1w(o, R, q, wy, last, yj2);
-            const double c01 = a01w(o, R, q, wy) - a01w(o, R, q, wu);
-            if (MODE == SM_HV) {
-              const double fd = dt * (a0y - a0u) + dt * (a1y - a1u) + dt * c01;
-              rhs += 0.5 * fd - tdt * a0y;
-              ex = tdt * a1y;
-            } else if (MODE == SM_CS) {
-              rhs += (0.5 * dt) * c01 - tdt * a0u;
-            } else {  // SM_MCS
-              const double th = tdt / dt;
-              const double fd = dt * ((a0y - a0u) + r0) + dt * (a1y - a1u) + dt * c01;
-              rhs += (th * dt) * c01 + (0.5 - th) * fd - tdt * a0u;
-            }
-          }
-        }
-        if (R.dir != 0.0) rhs = 0.0;  // the solve forces Dirichlet rows to 0
-        d[i * NL + jj] = rhs;
-        eb[i * NL + jj] = ex;
-        // I - tdt*A0 on this row; pivots from the factorisation.
-        const double aL = (R.dir != 0.0 || i == 0) ? 0.0 : -tdt * (R.psub + q.v * R.qsub);
-        const double m = -aL * rdprev;
-        const double rdi = rd[i * nv + j];
-        mreg[k] = m;
-        rdreg[k] = rdi;
-        E = rhs + m * E;
-        M = m * M;
-        rdprev = rdi;
-        // advance the windows by one row
+  for (int r = 0; r < 4; ++r) load_row32(u, s - 1 + r, nS, nv, jm, jc, jp, wu[r]);
+  if (HAS_Y) {
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          wu.v[0][b] = wu.v[1][b];
-          wu.v[1][b] = wu.v[2][b];
-        }
-        load_row(u, i + 2, nS, nv, jm, j, jp, wu.v[2]);
-        if (y2) {
+    for (int r = 0; r < 4; ++r) load_row32(y2, s - 1 + r, nS, nv, jm, jc, jp, wy[r]);
+  }
+  double2 ta0, tb0, ta1, tb1;  // table rows i and i+1
+  quad_row_load(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
+  quad_row_load(spr + clamp_row(s + 1) * kSpRow, jj, ta1, tb1);
+  double rd0 = rd[clamp_row(s) * nv + jc], rd1 = rd[clamp_row(s + 1) * nv + jc];
+  double M = 1.0, E = 0.0;
 #pragma unroll
-          for (int b = 0; b < 3; ++b) {
-            wy.v[0][b] = wy.v[1][b];
-            wy.v[1][b] = wy.v[2][b];
-          }
-          load_row(y2, i + 2, nS, nv, jm, j, jp, wy.v[2]);
+  for (int k = 0; k < MAXL; ++k) {
+    const int i = s + k;
+    const int ic = clamp_row(i);
+    // loads for two rows ahead
+    double2 ta2, tb2;
+    quad_row_load(spr + clamp_row(i + 2) * kSpRow, jj, ta2, tb2);
+    const double rd2 = rd[clamp_row(i + 2) * nv + jc];
+    double nu[3], ny[3];
+    load_row32(u, i + 3, nS, nv, jm, jc, jp, nu);
+    if (HAS_Y) load_row32(y2, i + 3, nS, nv, jm, jc, jp, ny);
+    const double uj2 = last ? u[ic * nv + jm2] : 0.0;
+    const double yj2 = (HAS_Y && last) ? y2[ic * nv + jm2] : 0.0;
+
+    const SpotRow R = quad_row_fields(ta0, tb0);
+    Win wU, wY;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        wU.v[r][b] = wu[r][b];
+        wY.v[r][b] = HAS_Y ? wy[r][b] : 0.0;
+      }
+    const bool dm = i == 1, dp = o.barrier && i + 1 == nS - 1;
+    double rhs = 0.0, ex = 0.0;
+    if (MODE == SM_IMPL) {
+      rhs = wU.v[1][1] + dt * (R.pr + q.v * R.qr);
+    } else {
+      const double a0u = a0w(R, q.v, wU, dm, dp);
+      const double a1u = a1w(o, R, q, wU, last, uj2);
+      const double r0 = R.pr + q.v * R.qr;  // zero on Dirichlet rows
+      const double fu = dt * (a0u + r0) + dt * a1u + dt * a01w(o, R, q, wU);
+      rhs = wU.v[1][1] + fu;
+      if (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) ex = tdt * a1u;
+      if (MODE == SM_FIRST) {
+        rhs -= tdt * a0u;
+      } else {
+        // Operators are linear: A (Y2 - U) = A Y2 - A U.
+        const double a0y = a0w(R, q.v, wY, dm, dp);
+        const double a1y = a1w(o, R, q, wY, last, yj2);
+        const double c01 = a01w(o, R, q, wY) - a01w(o, R, q, wU);
+        if (MODE == SM_HV) {
+          const double fd = dt * (a0y - a0u) + dt * (a1y - a1u) + dt * c01;
+          rhs += 0.5 * fd - tdt * a0y;
+          ex = tdt * a1y;
+        } else if (MODE == SM_CS) {
+          rhs += (0.5 * dt) * c01 - tdt * a0u;
+        } else {  // SM_MCS
+          const double th = tdt / dt;
+          const double fd = dt * ((a0y - a0u) + r0) + dt * (a1y - a1u) + dt * c01;
+          rhs += (th * dt) * c01 + (0.5 - th) * fd - tdt * a0u;
         }
       }
     }
+    if (R.dir != 0.0) rhs = 0.0;  // the solve forces Dirichlet rows to 0
+    // I - tdt*A0 on this row, scaled by the pivot.
+    const double aL = (R.dir != 0.0 || i == 0) ? 0.0 : -tdt * (R.psub + q.v * R.qsub);
+    const double cL = (R.dir != 0.0 || i >= nS - 1) ? 0.0 : -tdt * (R.psup + q.v * R.qsup);
+    const double dsc = rd0 * rhs;
+    mu[k] = -aL * rd0;
+    nreg[k] = -cL * rd0;
+    db[k * THREADS + tid] = dsc;
+    eb[k * THREADS + tid] = ex;
+    if (i < e) {
+      E = dsc + mu[k] * E;
+      M = mu[k] * M;
+    }
+    // rotate the rings
+    ta0 = ta1; tb0 = tb1; ta1 = ta2; tb1 = tb2;
+    rd0 = rd1; rd1 = rd2;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      wu[0][b] = wu[1][b]; wu[1][b] = wu[2][b]; wu[2][b] = wu[3][b]; wu[3][b] = nu[b];
+      if (HAS_Y) {
+        wy[0][b] = wy[1][b]; wy[1][b] = wy[2][b]; wy[2][b] = wy[3][b]; wy[3][b] = ny[b];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
   }
   cM[ch * NL + jj] = active ? M : 1.0;
   cE[ch * NL + jj] = active ? E : 0.0;
   __syncthreads();
   affine_scan<THREADS, NL, false>(cM, cE, cIn, TPL);
   __syncthreads();
-  // Forward fix-up, then the local backward recurrence x = rd y + n x'.
-  if (active) {
+  // Forward fix-up, then the local backward recurrence x = y' + n x'.
+  {
     double yv = cIn[ch * NL + jj];
 #pragma unroll
     for (int k = 0; k < MAXL; ++k) {
-      const int64_t i = s + k;
-      if (i < e) {
-        yv = d[i * NL + jj] + mreg[k] * yv;
-        d[i * NL + jj] = yv;
-      }
+      yv = db[k * THREADS + tid] + mu[k] * yv;
+      db[k * THREADS + tid] = yv;
     }
-    const double v = VT(c, o, VT_V, j);
-    M = 1.0;
-    E = 0.0;
+  }
+  M = 1.0;
+  E = 0.0;
 #pragma unroll
-    for (int k = MAXL - 1; k >= 0; --k) {
-      const int64_t i = s + k;
-      if (i < e) {
-        const double *rw = spr + i * kSpRow;
-        const double cL = (rw[12] != 0.0 || i == nS - 1) ? 0.0 : -o.tdt * (rw[2] + v * rw[5]);
-        const double n = -cL * rdreg[k];
-        mreg[k] = n;
-        E = rdreg[k] * d[i * NL + jj] + n * E;
-        M = n * M;
-      }
+  for (int k = MAXL - 1; k >= 0; --k) {
+    if (s + k < e) {
+      E = db[k * THREADS + tid] + nreg[k] * E;
+      M = nreg[k] * M;
     }
   }
   __syncthreads();  // everyone done reading cIn of the forward scan
@@ -467,90 +529,106 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   __syncthreads();
   affine_scan<THREADS, NL, true>(cM, cE, cIn, TPL);
   __syncthreads();
-  if (active) {
-    double x = cIn[ch * NL + jj];
-#pragma unroll
-    for (int k = MAXL - 1; k >= 0; --k) {
-      const int64_t i = s + k;
-      if (i < e) {
-        x = rdreg[k] * d[i * NL + jj] + mreg[k] * x;
-        d[i * NL + jj] = x;
-      }
-    }
-  }
-  __syncthreads();
-
-  // Phase 3: stage output x - tdt*A1(f), written with consecutive lanes on
-  // consecutive columns.
+  // Backward fix-up; stage output x - tdt*A1(f) written straight from the chunk.
   double *ob = out + o.off;
-  for (int64_t qi = tid; qi < nS * NL; qi += THREADS) {
-    const int64_t i = qi / NL, jq = j0 + qi % NL;
-    if (jq < nv) ob[i * nv + jq] = d[qi] - eb[qi];
+  double x = cIn[ch * NL + jj];
+#pragma unroll
+  for (int k = MAXL - 1; k >= 0; --k) {
+    const int i = s + k;
+    if (i < e) {
+      x = db[k * THREADS + tid] + nreg[k] * x;
+      if (col_ok) ob[i * nv + j] = x - eb[k * THREADS + tid];
+    }
   }
 }
 
 // ---- variance sweep -------------------------------------------------------------
 // Lines along the variance axis are contiguous.  A work-group of 256 threads
-// handles 256/TPL adjacent spot rows of one option; the TPL lanes of a line
-// (a wave or a segment of one) each own a contiguous chunk of L <= MAXL
-// positions in registers, and the chunk carries of both triangular sweeps
-// are combined with a segmented wave scan of affine maps (no LDS, no
-// work-group barrier).  Lines with S != 0 share the 1-D factorisation
-// (VT_M, VT_N, VT_RD, fold_f); the S = 0 line is diagonal.
+// handles G = 256/TPL adjacent spot rows of one option.  The rows are loaded
+// with coalesced reads into LDS, in a chunked layout: position j of a line
+// sits at (j / L) * P + j % L, with L = ceil(nv / TPL) positions per chunk
+// and an odd chunk pitch P, so that the TPL lanes of a line reading chunk
+// position k hit distinct banks.  The factorisation tables (M, N, RD, shared
+// by every S != 0 line of the option) are staged the same way.  Each lane
+// then solves its chunk in registers; the chunk carries of both triangular
+// sweeps are combined with a segmented wave scan of affine maps, the result
+// goes back to LDS and is stored with coalesced writes.  The S = 0 line is
+// diagonal.
 template <int MODE, int TPL, int MAXL>
 __global__ void __launch_bounds__(256)
 var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ W,
                  double *__restrict__ out) {
+  extern __shared__ double lds[];
+  constexpr int G = 256 / TPL;
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const HOpt o = c.opts[t.opt];
-  const int ln = threadIdx.x % TPL;
-  const int64_t i = t.start + threadIdx.x / TPL;
-  const int64_t nS = o.nS, nv = o.nv;
+  const int64_t nS = o.nS, nv = o.nv, i0 = t.start;
   const int L = (int)((nv + TPL - 1) / TPL);
-  const int64_t s = (int64_t)ln * L;
-  const int64_t e = s + L < nv ? s + L : nv;
-  const bool row_ok = i < nS;
-  const bool active = row_ok && s < e;
+  const int P = L | 1;
+  const int LP = TPL * P + 1;  // line pitch (odd)
+  double *tM = lds, *tN = tM + LP, *tR = tN + LP, *lines = tR + LP;
+  const int tid = threadIdx.x;
+  const int rows = (int)(nS - i0 < G ? nS - i0 : G);
   const double *w = W + o.off;
+  // Stage the tables and the rows.
+  for (int64_t jq = tid; jq < nv; jq += 256) {
+    const int pos = (int)(jq / L) * P + (int)(jq % L);
+    tM[pos] = VT(c, o, VT_M, jq);
+    tN[pos] = VT(c, o, VT_N, jq);
+    tR[pos] = VT(c, o, VT_RD, jq);
+  }
+  {
+    const int64_t tot = (int64_t)rows * nv;
+    int64_t g = 0, jq = tid;
+    while (jq >= nv) { jq -= nv; ++g; }
+    for (int64_t qq = tid; qq < tot; qq += 256) {
+      const int64_t i = i0 + g;
+      double val = w[i * nv + jq];
+      if (MODE == VM_IMPL) {
+        auto xw = [&](int64_t a, int64_t b) { return w[a * nv + b]; };
+        val += o.dt * a01_dot(c, o, i, jq, xw);
+      }
+      lines[g * LP + (int)(jq / L) * P + (int)(jq % L)] = val;
+      jq += 256;
+      while (jq >= nv) { jq -= nv; ++g; }
+    }
+  }
+  __syncthreads();
+  const int ln = tid % TPL, g = tid / TPL;
+  const int64_t i = i0 + g;
+  const int64_t sj = (int64_t)ln * L;
+  const int cnt = (int)(sj >= nv ? 0 : (sj + L <= nv ? L : nv - sj));
+  const bool row_ok = g < rows;
   const bool diag = row_ok && SP(c, o, SP_S, i) == 0.0;
-  const bool fact = active && !diag;
+  const bool fact = row_ok && !diag && cnt > 0;
+  double *lp = lines + g * LP + ln * P;
   double d[MAXL], mreg[MAXL], nreg[MAXL], rreg[MAXL];
   double dl = 0.0;  // this lane's last value (for the bottom fold)
 #pragma unroll
   for (int k = 0; k < MAXL; ++k) {
-    const int64_t j = s + k;
-    double val = 0.0;
-    if (active && j < e) {
-      if (MODE == VM_IMPL) {
-        auto xw = [&](int64_t a, int64_t b) { return w[a * nv + b]; };
-        val = w[i * nv + j] + o.dt * a01_dot(c, o, i, j, xw);
-      } else {
-        val = w[i * nv + j];
-      }
-    }
-    d[k] = val;
-    if (active && j == e - 1) dl = val;
+    const bool in = row_ok && k < cnt;
+    d[k] = in ? lp[k] : 0.0;
+    mreg[k] = in ? tM[ln * P + k] : 0.0;
+    nreg[k] = in ? tN[ln * P + k] : 0.0;
+    rreg[k] = in ? tR[ln * P + k] : 0.0;
+    if (in && k == cnt - 1) dl = d[k];
   }
   // Bottom fold of the right-hand side: d[nv-1] += f d[nv-2].
   const double prev_last = __shfl_up(dl, 1, TPL);
-  if (fact && e == nv) {
+  if (fact && sj + cnt == nv) {
 #pragma unroll
     for (int k = 0; k < MAXL; ++k)
-      if (s + k == nv - 1) d[k] += o.fold_f * (k > 0 ? d[k > 0 ? k - 1 : 0] : prev_last);
+      if (k == cnt - 1) d[k] += o.fold_f * (k > 0 ? d[k > 0 ? k - 1 : 0] : prev_last);
   }
   // Local forward recurrence y = d + m y.
   double M = 1.0, E = 0.0;
+  if (fact) {
 #pragma unroll
-  for (int k = 0; k < MAXL; ++k) {
-    const int64_t j = s + k;
-    if (fact && j < e) {
-      const double m = VT(c, o, VT_M, j);
-      mreg[k] = m;
-      nreg[k] = VT(c, o, VT_N, j);
-      rreg[k] = VT(c, o, VT_RD, j);
-      E = d[k] + m * E;
-      M = m * M;
-    }
+    for (int k = 0; k < MAXL; ++k)
+      if (k < cnt) {
+        E = d[k] + mreg[k] * E;
+        M = mreg[k] * M;
+      }
   }
   // Segmented inclusive prefix scan, then the carry entering each chunk.
 #pragma unroll
@@ -569,14 +647,14 @@ var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict
     double y = carry;
 #pragma unroll
     for (int k = 0; k < MAXL; ++k)
-      if (s + k < e) {
+      if (k < cnt) {
         y = d[k] + mreg[k] * y;
         d[k] = y;
       }
     // Local backward recurrence x = rd y + n x'.
 #pragma unroll
     for (int k = MAXL - 1; k >= 0; --k)
-      if (s + k < e) {
+      if (k < cnt) {
         E = rreg[k] * d[k] + nreg[k] * E;
         M = nreg[k] * M;
       }
@@ -592,23 +670,34 @@ var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict
   }
   double after = __shfl_down(E, 1, TPL);
   if (ln == TPL - 1) after = 0.0;
-  double *ob = out + o.off + i * nv;
   if (fact) {
     double x = after;
 #pragma unroll
     for (int k = MAXL - 1; k >= 0; --k)
-      if (s + k < e) {
+      if (k < cnt) {
         x = rreg[k] * d[k] + nreg[k] * x;
         d[k] = x;
       }
-  } else if (active) {
+  } else if (diag) {
 #pragma unroll
     for (int k = 0; k < MAXL; ++k) d[k] *= o.rdS0;
   }
-  if (active) {
+  if (row_ok) {
 #pragma unroll
     for (int k = 0; k < MAXL; ++k)
-      if (s + k < e) ob[s + k] = d[k];
+      if (k < cnt) lp[k] = d[k];
+  }
+  __syncthreads();
+  double *ob = out + o.off + i0 * nv;
+  {
+    const int64_t tot = (int64_t)rows * nv;
+    int64_t gq = 0, jq = tid;
+    while (jq >= nv) { jq -= nv; ++gq; }
+    for (int64_t qq = tid; qq < tot; qq += 256) {
+      ob[qq] = lines[gq * LP + (int)(jq / L) * P + (int)(jq % L)];
+      jq += 256;
+      while (jq >= nv) { jq -= nv; ++gq; }
+    }
   }
 }
 
@@ -916,10 +1005,10 @@ template <int MODE, int T, int ML = kMAXLs>
 int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                   hipStream_t s) {
   constexpr int TPL = T / kNLs;
-  const size_t lds = sizeof(double) * (2 * hb->nSmax * kNLs + 3 * TPL * kNLs);
+  const size_t lds = sizeof(double) * (2 * ML * T + 3 * TPL * kNLs);
   FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, kNLs, ML>, lds));
   hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, kNLs, ML>), dim3(hb->nstiles), dim3(T), lds,
-                     s, ctx_of(hb), hb->d_stiles, U, Y, out, hb->nSmax);
+                     s, ctx_of(hb), hb->d_stiles, U, Y, out);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
@@ -948,7 +1037,11 @@ int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *ou
 
 template <int MODE, int TPL>
 int launch_var_t(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
-  hipLaunchKernelGGL((var_sweep_kernel<MODE, TPL, kMAXL>), dim3(hb->nvtiles), dim3(256), 0, s,
+  // LDS: three tables and 256/TPL lines, each TPL chunks of odd pitch + 1.
+  const int64_t L = (hb->nvmax + TPL - 1) / TPL;
+  const size_t lds = sizeof(double) * (3 + 256 / TPL) * (TPL * (L | 1) + 1);
+  FDADI_RET(allow_lds((const void *)var_sweep_kernel<MODE, TPL, kMAXL>, lds));
+  hipLaunchKernelGGL((var_sweep_kernel<MODE, TPL, kMAXL>), dim3(hb->nvtiles), dim3(256), lds, s,
                      ctx_of(hb), hb->d_vtiles, W, out);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
@@ -1060,10 +1153,10 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
                 ": grid needs at least 4 points per axis");
       return FDADI_ERR_ARG;
     }
-    if (q.nS > 2048 || q.nv > 2048) {
+    if (q.nS > 2048 || q.nv > 1024) {
       free_hb(hb);
       set_error("fdadi_hb_create: option " + std::to_string(k) +
-                ": the fused engine supports nS, nv <= 2048");
+                ": the fused engine supports nS <= 2048 and nv <= 1024");
       return FDADI_ERR_UNSUPPORTED;
     }
     HOpt &o = hb->h_opts[k];

@@ -199,6 +199,20 @@ def adi_prices(P, nS=256, nv=128, nt=200, scheme="cs", chunk=4096, grids=None):
     return out
 
 
+def within_call_bounds(P, y, slack=1.0):
+    """Rows whose price lies within the European call bounds
+    max(S0 - K exp(-rT), 0) <= C <= S0 (S0 = 100), widened by ``slack``.
+
+    The ADI schemes on a coarse label mesh are unstable for some
+    advection-dominated variance dynamics (large kappa*theta, small sigma);
+    such labels leave these bounds by orders of magnitude and are dropped
+    from the training set."""
+    P = np.asarray(P, dtype=float)
+    y = np.asarray(y, dtype=float)
+    lower = np.maximum(100.0 - 100.0 * P[:, 7] * np.exp(-P[:, 5] * P[:, 6]), 0.0)
+    return np.isfinite(y) & (y >= lower - slack) & (y <= 100.0 + slack)
+
+
 class SurrogatePricer(object):
     """Ensemble surrogate with a per-query gate and the FP64 ADI fallback."""
 

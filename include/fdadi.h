@@ -181,7 +181,8 @@ typedef struct {
   int32_t barrier;                    /* nonzero: Dirichlet 0 at S = Smax (up-and-out) */
   int32_t pad;
 } fdadi_heston_option;
-/* spots/vars are the meshes of all options concatenated (host). */
+/* spots/vars are the meshes of all options concatenated (host).  Each option
+ * needs 4 <= nS <= 2048 and 4 <= nv <= 1024 (FDADI_ERR_UNSUPPORTED beyond). */
 int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double *spots,
                     const double *vars, void *stream, fdadi_hb **out);
 int fdadi_hb_destroy(fdadi_hb *hb);

@@ -60,11 +60,16 @@ def test_gradients_match_reference(fd):
 def test_training_on_adi_labels_beats_baselines(fd):
     """The network trained on ADI labels must beat the best constant and the
     best linear model on held-out queries (what the surrogate gains)."""
-    from fd_adi_amd.surrogate import MLP, adi_prices, BOX_LO, BOX_HI
+    from fd_adi_amd.surrogate import MLP, adi_prices, within_call_bounds, BOX_LO, BOX_HI
     rng = np.random.default_rng(5)
-    n, nv = 4096 + 1024, 16
-    P = rng.uniform(BOX_LO, BOX_HI, (n, 8))
+    n0, nv = 4096 + 1024 + 64, 16
+    P = rng.uniform(BOX_LO, BOX_HI, (n0, 8))
     y = adi_prices(P, nS=48, nv=nv, nt=40)
+    ok = within_call_bounds(P, y)
+    # the coarse label mesh is unstable for a few advection-dominated rows
+    assert ok.mean() > 0.99, ok.mean()
+    P, y = P[ok][:4096 + 1024], y[ok][:4096 + 1024]
+    n = P.shape[0]
     tr, te = slice(0, 4096), slice(4096, n)
     mu, sd = y[tr].mean(), y[tr].std()
     m = MLP(BOX_LO, BOX_HI, mu, sd, seed=0, max_rows=1024)
@@ -78,3 +83,26 @@ def test_training_on_adi_labels_beats_baselines(fd):
     print("loss history", hist, "rmse mlp %.4f linear %.4f const %.4f" % (err_mlp, err_lin, err_const))
     assert hist[-1][1] < hist[0][1]
     assert err_mlp < 0.5 * err_lin < err_const
+
+
+def test_batched_labels_match_oracle(fd):
+    """The batched engine's labels on the surrogate's label meshes equal the
+    oracle's Craig-Sneyd solve on the same meshes (32 random rows)."""
+    from fd_adi_amd.surrogate import adi_prices, label_grids, BOX_LO, BOX_HI
+    from fd_adi_amd.utils import nearest_index
+    from oracle.engine import HestonParams, HestonOracle
+    rng = np.random.default_rng(11)
+    P = rng.uniform(BOX_LO, BOX_HI, (32, 8))
+    nS, nv, nt = 48, 16, 40
+    y = adi_prices(P, nS=nS, nv=nv, nt=nt)
+    S, V = label_grids(P, nS, nv)
+    want = []
+    for k, r in enumerate(P):
+        hp = HestonParams(spot=100.0, strike=100 * r[7], r=r[5], v0=r[4], T=r[6], kappa=r[0],
+                          theta=r[1], sigma=r[2], rho=r[3])
+        E = HestonOracle(hp, S[k], V[k])
+        out = E.solve_craigsneyd(nt, r[6] / nt, E.payoff(), 0.5)
+        want.append(out[nearest_index(S[k], 100.0), nearest_index(V[k], r[4])])
+    want = np.array(want)
+    ok = np.abs(want) < 1e3
+    npt.assert_allclose(y[ok], want[ok], rtol=1e-11, atol=1e-11)
