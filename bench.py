@@ -49,8 +49,13 @@ def config1():
     return single(2048, 1024, 2000, -0.7)
 
 
-def run_single(opt, scheme, theta, K, W, smoothing=0):
-    """Time K solves of one option (state reset from a device-resident copy)."""
+def run_single(opt, scheme, theta, K, W, smoothing=0, graph=False):
+    """Time K solves of one option (state reset from a device-resident copy).
+
+    Every solve rebuilds the factorisations (configure) and runs all nt
+    steps.  With ``graph`` the steps of a solve are one HIP graph replay
+    (captured once, untimed); the last timed solve launches its stages one by
+    one with a HIP event pair around each (the per-kernel durations)."""
     import numpy as np
     import fd_adi_amd
     hb = fd_adi_amd.HestonBatch([opt.params], [opt.spots], [opt.vars], [opt.dt])
@@ -58,6 +63,11 @@ def run_single(opt, scheme, theta, K, W, smoothing=0):
     hb.set_state([U0])
     init = hb.state_view(0).copy(True)          # payoff kept in HBM
     view = hb.state_view(0)
+    n = opt.nt - smoothing
+    if graph:
+        assert not smoothing
+        hb.configure(scheme, theta, 1.0)
+        hb.capture(n)
 
     def solve(timed=False):
         view.copy_from(init)
@@ -66,9 +76,10 @@ def run_single(opt, scheme, theta, K, W, smoothing=0):
             hb.configure("implicit", 1.0, 0.5)
             hb.step(2 * smoothing)
         hb.configure(scheme, theta, 1.0)
-        n = opt.nt - smoothing
         if timed:
             stages = hb.step_timed(n)
+        elif graph:
+            hb.replay()
         else:
             hb.step(n)
         return stages
@@ -148,6 +159,23 @@ print(json.dumps({"sec_per_step": (time.perf_counter() - t) / %d}))
     return sec
 
 
+def probe():
+    """Fixed dispatch sequence for counter runs (see --probe)."""
+    import fd_adi_amd
+    from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
+    opt = config1()
+    hb = fd_adi_amd.HestonBatch([opt.params], [opt.spots], [opt.vars], [opt.dt])
+    hb.set_state([opt.payoff()])
+    hb.configure("hv", 0.5, 1.0)
+    hb.step(20)
+    hb.sync()
+    x = SizedArrayPtr().alloc(1 << 25)
+    for _ in range(5):
+        x.timeseq_scalar_from_host(1.0)
+    fd_adi_amd._lib.sync()
+    log("probe: 20 HV steps (80 sweep launches) + 5 scale launches on 2^25 doubles done")
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 summary."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_config1.txt")
@@ -168,6 +196,11 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="profiling probe for rocprofv3 counter runs: 20 HV steps of the "
+                         "2048x1024 option, then 5 launches of the scale kernel on 2^25 "
+                         "doubles (reads and writes 256 MiB each: the copy reference); "
+                         "prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
@@ -177,11 +210,14 @@ def main():
     import numpy as np
     import fd_adi_amd  # noqa: F401
 
+    if args.probe:
+        probe()
+        return
     opt = config1()
     N = opt.spots.size * opt.vars.size
     log("config1: %dx%d, nt=%d, HV; warmup %d, timed %d" % (opt.spots.size, opt.vars.size,
                                                            opt.nt, args.warmup, args.steps))
-    wall, stages, price, _ = run_single(opt, "hv", 0.5, args.steps, args.warmup)
+    wall, stages, price, _ = run_single(opt, "hv", 0.5, args.steps, args.warmup, graph=True)
     ms_per = 1e3 * wall / args.steps
     value = args.steps / wall
     per_launch_ms = stages / opt.nt

@@ -8,7 +8,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfdadi.so")
+# FDADI_LIB names another build of the same library (e.g. one compiled for a
+# different architecture); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("FDADI_LIB") or os.path.join(_HERE, "libfdadi.so")
 
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 CANNOT_FOLD_I, CAN_FOLD_I, FOLDED_I = 0, 1, 2

@@ -192,6 +192,99 @@ __device__ __forceinline__ void li0_row(const Ctx &c, const HOpt &o, int64_t i, 
   if (i == o.nS - 1) cc = 0.0;
 }
 
+// ---- wave-level scans of affine maps ------------------------------------------
+// A map y -> E + M y is the pair (M, E); composing "earlier then later" gives
+// (M_l M_e, E_l + M_l E_e).  The scans run on DPP lane moves (no LDS round
+// trip): row_shr/row_shl within rows of 16 lanes, row_bcast:15/31 across rows
+// for prefixes, wave_shr/wave_shl for the exclusive shift.  Lanes with no
+// source keep the identity (M, E) = (1, 0).  Every lane of the wave must be
+// active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_or(double x, double old) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(x), CTRL, ROWS,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(x), CTRL, ROWS,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// One prefix level: combine with the map of lane ln - D (ln = lane within a
+// segment of W lanes); the DPP control already fetches from lane - D.
+template <int CTRL, int ROWS, int D, int W>
+__device__ __forceinline__ void prefix_level(double &M, double &E, int ln) {
+  double Mp = dpp_or<CTRL, ROWS>(M, 1.0), Ep = dpp_or<CTRL, ROWS>(E, 0.0);
+  if (W < 16 && ln < D) {
+    Mp = 1.0;
+    Ep = 0.0;
+  }
+  E = E + M * Ep;
+  M = M * Mp;
+}
+
+// Inclusive prefix scan over segments of W lanes (W a power of two <= 64).
+template <int W>
+__device__ __forceinline__ void wave_prefix_scan(double &M, double &E, int ln) {
+  if (W > 1) prefix_level<0x111, 0xF, 1, W>(M, E, ln);
+  if (W > 2) prefix_level<0x112, 0xF, 2, W>(M, E, ln);
+  if (W > 4) prefix_level<0x114, 0xF, 4, W>(M, E, ln);
+  if (W > 8) prefix_level<0x118, 0xF, 8, W>(M, E, ln);
+  if (W > 16) prefix_level<0x142, 0xA, 16, W>(M, E, ln);  // row_bcast:15 into rows 1, 3
+  if (W > 32) prefix_level<0x143, 0xC, 32, W>(M, E, ln);  // row_bcast:31 into rows 2, 3
+}
+
+// Inclusive suffix scan over segments of W lanes: maps of higher lanes are
+// applied first.  Within rows by DPP row_shl (after those levels a lane holds
+// the maps from itself to the end of its row).  Across rows, mirroring
+// row_bcast of the prefix scan: at D = 16 rows 0 and 2 take the first lane of
+// the next row (which holds that whole row), at D = 32 rows 0 and 1 take lane
+// 32 (which then holds rows 2-3); by ds_bpermute.
+template <int CTRL, int D, int W>
+__device__ __forceinline__ void suffix_level(double &M, double &E, int ln) {
+  double Mn, En;
+  bool take;
+  if (D < 16) {
+    Mn = dpp_or<CTRL, 0xF>(M, 1.0);
+    En = dpp_or<CTRL, 0xF>(E, 0.0);
+    take = ln + D < W;
+  } else {
+    const int lane = threadIdx.x % 64;
+    const int src = D == 16 ? (lane | 15) + 1 : 32;
+    Mn = __shfl(M, src & 63, 64);
+    En = __shfl(E, src & 63, 64);
+    take = D == 16 ? ((lane >> 4) & 1) == 0 : lane < 32;
+  }
+  if (!take) {
+    Mn = 1.0;
+    En = 0.0;
+  }
+  E = E + M * En;
+  M = M * Mn;
+}
+
+template <int W>
+__device__ __forceinline__ void wave_suffix_scan(double &M, double &E, int ln) {
+  if (W > 1) suffix_level<0x101, 1, W>(M, E, ln);
+  if (W > 2) suffix_level<0x102, 2, W>(M, E, ln);
+  if (W > 4) suffix_level<0x104, 4, W>(M, E, ln);
+  if (W > 8) suffix_level<0x108, 8, W>(M, E, ln);
+  if (W > 16) suffix_level<0, 16, W>(M, E, ln);
+  if (W > 32) suffix_level<0, 32, W>(M, E, ln);
+}
+
+// Value of lane ln - 1 of the segment (0 for its first lane).
+template <int W>
+__device__ __forceinline__ double from_prev_lane(double x, int ln) {
+  const double v = dpp_or<0x138, 0xF>(x, 0.0);  // wave_shr:1
+  return ln == 0 ? 0.0 : v;
+}
+
+// Value of lane ln + 1 of the segment (0 for its last lane).
+template <int W>
+__device__ __forceinline__ double from_next_lane(double x, int ln) {
+  const double v = dpp_or<0x130, 0xF>(x, 0.0);  // wave_shl:1
+  return ln == W - 1 ? 0.0 : v;
+}
+
 // Exclusive scan of affine maps y -> E + M y along the TPL chunks of each of
 // the NL lines of a tile.  Entries live in LDS at [c*NL + l].  Forward
 // (REV=false): carry(0)=0, carry(c+1)=E[c]+M[c]*carry(c).  Backward
@@ -216,17 +309,10 @@ __device__ void affine_scan(const double *cM, const double *cE, double *cIn, int
     }
     // Inclusive wave scan of (M, E) (later maps applied after earlier ones).
     double M = Mloc, E = Eloc;
-#pragma unroll
-    for (int d = 1; d < W; d <<= 1) {
-      const double Mo = __shfl_up(M, d, W), Eo = __shfl_up(E, d, W);
-      if (lane >= d) {
-        E = E + M * Eo;
-        M = M * Mo;
-      }
-    }
+    wave_prefix_scan<W>(M, E, lane);
     // Exclusive: the carry entering this lane's first entry.
-    double carry = __shfl_up(E, 1, W);
-    if (lane == 0) carry = 0.0;
+    const double carry0 = from_prev_lane<W>(E, lane);
+    double carry = carry0;
     for (int q = 0; q < K; ++q) {
       const int s = lane * K + q;
       if (s < tpl) {
@@ -239,66 +325,52 @@ __device__ void affine_scan(const double *cM, const double *cE, double *cIn, int
 }
 
 // ---- spot sweep ---------------------------------------------------------------
-// One 128-byte row of per-node spot coefficients (AoS copy of the spot table):
-// [psub pmid psup qsub qmid qsup pr qr wsub wmid wsup S dir - - -].
+// One 128-byte row of per-node spot coefficients, built from the spot table
+// with the boundary handling folded in:
+// [psub' pmid psup' qsub' | qmid qsup' pr qr | S*wsub S*wmid S*wsup snz | 0 0 0 0]
+// where ' marks band entries zeroed when they multiply a Dirichlet neighbour
+// (row 0 for row 1; row nS-1 for row nS-2 under a barrier) -- the explicit
+// apply forces those neighbours to 0, and in the solve the coupling to a
+// Dirichlet row (whose value is 0) contributes nothing -- and snz = [S != 0].
+// Dirichlet rows have all-zero bands.
 constexpr int kSpRow = 16;
 
 struct SpotRow {
-  double psub, pmid, psup, qsub, qmid, qsup, pr, qr, wsub, wmid, wsup, S, dir;
+  double psub, pmid, psup, qsub, qmid, qsup, pr, qr, wsub, wmid, wsup, snz;
 };
-
-__device__ __forceinline__ SpotRow load_spot_row(const double *__restrict__ spr) {
-  const double2 *p = reinterpret_cast<const double2 *>(spr);
-  const double2 a = p[0], b = p[1], c2 = p[2], d = p[3], e = p[4], f = p[5], g = p[6];
-  SpotRow r;
-  r.psub = a.x; r.pmid = a.y; r.psup = b.x; r.qsub = b.y; r.qmid = c2.x; r.qsup = c2.y;
-  r.pr = d.x; r.qr = d.y; r.wsub = e.x; r.wmid = e.y; r.wsup = f.x; r.S = f.y; r.dir = g.x;
-  return r;
-}
 
 // 3x3 neighbourhood of a field around (i, j): rows i-1, i, i+1; cols j-1, j, j+1.
 struct Win {
   double v[3][3];
 };
 
-__device__ __forceinline__ void load_row(const double *__restrict__ f, int64_t i, int64_t nS,
-                                         int64_t nv, int64_t jm, int64_t j, int64_t jp,
-                                         double out[3]) {
-  const int64_t ii = i < 0 ? 0 : (i >= nS ? nS - 1 : i);
-  const double *r = f + ii * nv;
-  out[0] = r[jm];
-  out[1] = r[j];
-  out[2] = r[jp];
-}
-
 struct VarCol {  // variance coefficients of column j (constant along a spot line)
   double gsub, gmid, gsup, wv0, wv1, wv2, v;
+  double rsv;     // rho * sigma * v
+  double lastGz;  // A1[nv-1, nv-3] on the last column, else 0
 };
 
-// (A0 f)(i,j) from the window; 0 on Dirichlet rows, neighbours on Dirichlet
-// rows forced to 0.
-__device__ __forceinline__ double a0w(const SpotRow &R, double v, const Win &w, bool dm,
-                                      bool dp) {
-  if (R.dir != 0.0) return 0.0;
+// (A0 f)(i,j) from the window (Dirichlet handling is in the bands).
+__device__ __forceinline__ double a0w(const SpotRow &R, double v, const Win &w) {
   const double sub = R.psub + v * R.qsub, mid = R.pmid + v * R.qmid, sup = R.psup + v * R.qsup;
-  return sub * (dm ? 0.0 : w.v[0][1]) + mid * w.v[1][1] + sup * (dp ? 0.0 : w.v[2][1]);
+  return sub * w.v[0][1] + mid * w.v[1][1] + sup * w.v[2][1];
 }
 
+// (A1 f)(i,j); fjm2 = f(i, j-2) (used on the last column only).
 __device__ __forceinline__ double a1w(const HOpt &o, const SpotRow &R, const VarCol &q,
-                                      const Win &w, bool last, double fjm2) {
-  const double snz = R.S != 0.0 ? 1.0 : 0.0;
-  double acc = (snz * q.gmid - 0.5 * o.r) * w.v[1][1];
-  acc += snz * (q.gsub * w.v[1][0] + q.gsup * w.v[1][2] + (last ? o.Gz * fjm2 : 0.0));
+                                      const Win &w, double fjm2) {
+  double acc = (R.snz * q.gmid - 0.5 * o.r) * w.v[1][1];
+  acc += R.snz * (q.gsub * w.v[1][0] + q.gsup * w.v[1][2] + q.lastGz * fjm2);
   return acc;
 }
 
-__device__ __forceinline__ double a01w(const HOpt &o, const SpotRow &R, const VarCol &q,
-                                       const Win &w) {
+// (A01 f)(i,j) = rho sigma S v (ws x wv) . f, zero on boundary rows and columns
+// (their weights are zero).
+__device__ __forceinline__ double a01w(const SpotRow &R, const VarCol &q, const Win &w) {
   const double r0 = q.wv0 * w.v[0][0] + q.wv1 * w.v[0][1] + q.wv2 * w.v[0][2];
   const double r1 = q.wv0 * w.v[1][0] + q.wv1 * w.v[1][1] + q.wv2 * w.v[1][2];
   const double r2 = q.wv0 * w.v[2][0] + q.wv1 * w.v[2][1] + q.wv2 * w.v[2][2];
-  const double coef = ((o.rho * o.sigma) * R.S) * q.v;
-  return coef * (R.wsub * r0 + R.wmid * r1 + R.wsup * r2);
+  return q.rsv * (R.wsub * r0 + R.wmid * r1 + R.wsup * r2);
 }
 
 // Work-group index -> tile index so that consecutive tiles (which share cache
@@ -317,9 +389,10 @@ __device__ __forceinline__ double quad_bcast(double x) {
   return __hiloint2double(hi, lo);
 }
 
-// Spot-table row shared by a quad of lanes: lane jj holds doubles [4jj, 4jj+4)
-// of the 128-byte row (two 16-byte loads, so a quad reads each row once) and
-// the fields are broadcast.  Every lane of the quad must execute this.
+// Spot-table row shared by a quad of lanes: lane jj of the quad loads doubles
+// [4jj, 4jj+4) of the 128-byte row (two 16-byte loads), so a quad reads each
+// row once, and the fields are broadcast.  Every lane of the quad must
+// execute quad_row_fields.
 __device__ __forceinline__ void quad_row_load(const double *__restrict__ spr, int jj, double2 &a,
                                               double2 &b) {
   const double2 *p = reinterpret_cast<const double2 *>(spr + 4 * jj);
@@ -334,20 +407,19 @@ __device__ __forceinline__ SpotRow quad_row_fields(double2 a, double2 b) {
   r.qmid = quad_bcast<1>(a.x); r.qsup = quad_bcast<1>(a.y);
   r.pr = quad_bcast<1>(b.x);   r.qr = quad_bcast<1>(b.y);
   r.wsub = quad_bcast<2>(a.x); r.wmid = quad_bcast<2>(a.y);
-  r.wsup = quad_bcast<2>(b.x); r.S = quad_bcast<2>(b.y);
-  r.dir = quad_bcast<3>(a.x);
+  r.wsup = quad_bcast<2>(b.x); r.snz = quad_bcast<2>(b.y);
   return r;
 }
 
-// Row r (clamped to the grid) of a field at columns jm, j, jp; 32-bit offsets
-// (an option's grid has at most 2048 x 1024 nodes).
-__device__ __forceinline__ void load_row32(const double *__restrict__ f, int r, int nS, int nv,
-                                           int jm, int j, int jp, double out[3]) {
-  const int rr = r < 0 ? 0 : (r >= nS ? nS - 1 : r);
-  const double *p = f + rr * nv;
-  out[0] = p[jm];
-  out[1] = p[j];
-  out[2] = p[jp];
+// Row rr (already clamped to the grid) of a field at columns jm, j, jp.
+// Offsets are 32-bit unsigned (an option's grid has at most 2048 x 1024
+// nodes), so the loads address from a scalar base.
+__device__ __forceinline__ void load_row32(const double *__restrict__ f, unsigned rr, unsigned nv,
+                                           unsigned jm, unsigned j, unsigned jp, double out[3]) {
+  const unsigned b = rr * nv;
+  out[0] = f[b + jm];
+  out[1] = f[b + j];
+  out[2] = f[b + jp];
 }
 
 // A spot sweep: one work-group per (option, NL = 4 adjacent variance
@@ -395,45 +467,51 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   const double *y2 = HAS_Y ? Y + o.off : nullptr;
   const double *rd = c.rd0 + o.off;
   const double *spr = c.spr + o.so * kSpRow;
-  auto clamp_row = [&](int i) { return i < 0 ? 0 : (i >= nS ? nS - 1 : i); };
+  const unsigned unv = (unsigned)nv;
+  auto clamp_row = [&](int i) { return (unsigned)(i < 0 ? 0 : (i >= nS ? nS - 1 : i)); };
 
   VarCol q;
   q.gsub = VT(c, o, VT_GSUB, jc); q.gmid = VT(c, o, VT_GMID, jc); q.gsup = VT(c, o, VT_GSUP, jc);
   q.wv0 = VT(c, o, VT_WSUB, jc); q.wv1 = VT(c, o, VT_WMID, jc); q.wv2 = VT(c, o, VT_WSUP, jc);
   q.v = VT(c, o, VT_V, jc);
-  const int jm = jc > 0 ? jc - 1 : 0, jp = jc < nv - 1 ? jc + 1 : nv - 1;
-  const int jm2 = jc > 1 ? jc - 2 : 0;
+  q.rsv = (o.rho * o.sigma) * q.v;
   const bool last = jc == nv - 1;
+  q.lastGz = last ? o.Gz : 0.0;
+  const unsigned ujc = (unsigned)jc;
+  const unsigned jm = jc > 0 ? ujc - 1 : 0u, jp = jc < nv - 1 ? ujc + 1 : ujc;
+  const unsigned jm2 = jc > 1 ? ujc - 2 : 0u;
   const double tdt = o.tdt, dt = o.dt;
+  const int dir_hi = o.barrier ? nS - 1 : -1;  // Dirichlet rows: 0 and dir_hi
 
   // Phase 1: walk the chunk with 3x3 windows, form the fused right-hand side
   // (and the next stage's explicit term), and run the local forward recurrence.
   double mu[MAXL], nreg[MAXL];
   double wu[4][3], wy[4][3];  // rows i-1, i, i+1 and i+2 (in flight)
 #pragma unroll
-  for (int r = 0; r < 4; ++r) load_row32(u, s - 1 + r, nS, nv, jm, jc, jp, wu[r]);
+  for (int r = 0; r < 4; ++r) load_row32(u, clamp_row(s - 1 + r), unv, jm, ujc, jp, wu[r]);
   if (HAS_Y) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) load_row32(y2, s - 1 + r, nS, nv, jm, jc, jp, wy[r]);
+    for (int r = 0; r < 4; ++r) load_row32(y2, clamp_row(s - 1 + r), unv, jm, ujc, jp, wy[r]);
   }
   double2 ta0, tb0, ta1, tb1;  // table rows i and i+1
   quad_row_load(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
   quad_row_load(spr + clamp_row(s + 1) * kSpRow, jj, ta1, tb1);
-  double rd0 = rd[clamp_row(s) * nv + jc], rd1 = rd[clamp_row(s + 1) * nv + jc];
+  double rd0 = rd[clamp_row(s) * unv + ujc], rd1 = rd[clamp_row(s + 1) * unv + ujc];
   double M = 1.0, E = 0.0;
 #pragma unroll
   for (int k = 0; k < MAXL; ++k) {
     const int i = s + k;
-    const int ic = clamp_row(i);
+    const unsigned ic = clamp_row(i);
     // loads for two rows ahead
     double2 ta2, tb2;
-    quad_row_load(spr + clamp_row(i + 2) * kSpRow, jj, ta2, tb2);
-    const double rd2 = rd[clamp_row(i + 2) * nv + jc];
+    const unsigned i2 = clamp_row(i + 2), i3 = clamp_row(i + 3);
+    quad_row_load(spr + i2 * kSpRow, jj, ta2, tb2);
+    const double rd2 = rd[i2 * unv + ujc];
     double nu[3], ny[3];
-    load_row32(u, i + 3, nS, nv, jm, jc, jp, nu);
-    if (HAS_Y) load_row32(y2, i + 3, nS, nv, jm, jc, jp, ny);
-    const double uj2 = last ? u[ic * nv + jm2] : 0.0;
-    const double yj2 = (HAS_Y && last) ? y2[ic * nv + jm2] : 0.0;
+    load_row32(u, i3, unv, jm, ujc, jp, nu);
+    if (HAS_Y) load_row32(y2, i3, unv, jm, ujc, jp, ny);
+    const double uj2 = u[ic * unv + jm2];
+    const double yj2 = HAS_Y ? y2[ic * unv + jm2] : 0.0;
 
     const SpotRow R = quad_row_fields(ta0, tb0);
     Win wU, wY;
@@ -444,24 +522,23 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
         wU.v[r][b] = wu[r][b];
         wY.v[r][b] = HAS_Y ? wy[r][b] : 0.0;
       }
-    const bool dm = i == 1, dp = o.barrier && i + 1 == nS - 1;
     double rhs = 0.0, ex = 0.0;
     if (MODE == SM_IMPL) {
       rhs = wU.v[1][1] + dt * (R.pr + q.v * R.qr);
     } else {
-      const double a0u = a0w(R, q.v, wU, dm, dp);
-      const double a1u = a1w(o, R, q, wU, last, uj2);
+      const double a0u = a0w(R, q.v, wU);
+      const double a1u = a1w(o, R, q, wU, uj2);
       const double r0 = R.pr + q.v * R.qr;  // zero on Dirichlet rows
-      const double fu = dt * (a0u + r0) + dt * a1u + dt * a01w(o, R, q, wU);
+      const double fu = dt * (a0u + r0) + dt * a1u + dt * a01w(R, q, wU);
       rhs = wU.v[1][1] + fu;
       if (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) ex = tdt * a1u;
       if (MODE == SM_FIRST) {
         rhs -= tdt * a0u;
       } else {
         // Operators are linear: A (Y2 - U) = A Y2 - A U.
-        const double a0y = a0w(R, q.v, wY, dm, dp);
-        const double a1y = a1w(o, R, q, wY, last, yj2);
-        const double c01 = a01w(o, R, q, wY) - a01w(o, R, q, wU);
+        const double a0y = a0w(R, q.v, wY);
+        const double a1y = a1w(o, R, q, wY, yj2);
+        const double c01 = a01w(R, q, wY) - a01w(R, q, wU);
         if (MODE == SM_HV) {
           const double fd = dt * (a0y - a0u) + dt * (a1y - a1u) + dt * c01;
           rhs += 0.5 * fd - tdt * a0y;
@@ -475,10 +552,11 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
         }
       }
     }
-    if (R.dir != 0.0) rhs = 0.0;  // the solve forces Dirichlet rows to 0
-    // I - tdt*A0 on this row, scaled by the pivot.
-    const double aL = (R.dir != 0.0 || i == 0) ? 0.0 : -tdt * (R.psub + q.v * R.qsub);
-    const double cL = (R.dir != 0.0 || i >= nS - 1) ? 0.0 : -tdt * (R.psup + q.v * R.qsup);
+    if (i == 0 || i == dir_hi) rhs = 0.0;  // the solve forces Dirichlet rows to 0
+    // I - tdt*A0 on this row (zero off-diagonals on Dirichlet rows and at the
+    // line ends come from the bands), scaled by the pivot.
+    const double aL = -tdt * (R.psub + q.v * R.qsub);
+    const double cL = -tdt * (R.psup + q.v * R.qsup);
     const double dsc = rd0 * rhs;
     mu[k] = -aL * rd0;
     nreg[k] = -cL * rd0;
@@ -566,31 +644,76 @@ var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict
   const int L = (int)((nv + TPL - 1) / TPL);
   const int P = L | 1;
   const int LP = TPL * P + 1;  // line pitch (odd)
+  const float inv_L = 1.0f / (float)L;
+  // LDS position of line position jq: jq + (P - L) * (jq / L)
+  auto pos_of = [&](int jq) {
+    int dq = (int)((float)jq * inv_L);
+    const int rq = jq - dq * L;
+    if (rq < 0) --dq;
+    if (rq >= L) ++dq;
+    return jq + (P - L) * dq;
+  };
   double *tM = lds, *tN = tM + LP, *tR = tN + LP, *lines = tR + LP;
   const int tid = threadIdx.x;
   const int rows = (int)(nS - i0 < G ? nS - i0 : G);
   const double *w = W + o.off;
-  // Stage the tables and the rows.
-  for (int64_t jq = tid; jq < nv; jq += 256) {
-    const int pos = (int)(jq / L) * P + (int)(jq % L);
-    tM[pos] = VT(c, o, VT_M, jq);
-    tN[pos] = VT(c, o, VT_N, jq);
-    tR[pos] = VT(c, o, VT_RD, jq);
-  }
+  // Stage the tables and the rows.  Loop bounds are compile-time (a line has
+  // at most TPL*MAXL positions, a tile at most 256*MAXL) so that every load
+  // of a thread is issued before the first LDS write.
+  constexpr int TQ = (TPL * MAXL + 255) / 256;
   {
-    const int64_t tot = (int64_t)rows * nv;
-    int64_t g = 0, jq = tid;
-    while (jq >= nv) { jq -= nv; ++g; }
-    for (int64_t qq = tid; qq < tot; qq += 256) {
-      const int64_t i = i0 + g;
-      double val = w[i * nv + jq];
-      if (MODE == VM_IMPL) {
-        auto xw = [&](int64_t a, int64_t b) { return w[a * nv + b]; };
-        val += o.dt * a01_dot(c, o, i, jq, xw);
+    double m_[TQ], n_[TQ], r_[TQ];
+#pragma unroll
+    for (int a = 0; a < TQ; ++a) {
+      const int jq = tid + 256 * a;
+      const int jl = jq < nv ? jq : 0;
+      m_[a] = VT(c, o, VT_M, jl);
+      n_[a] = VT(c, o, VT_N, jl);
+      r_[a] = VT(c, o, VT_RD, jl);
+    }
+#pragma unroll
+    for (int a = 0; a < TQ; ++a) {
+      const int jq = tid + 256 * a;
+      if (jq < nv) {
+        const int pos = pos_of(jq);
+        tM[pos] = m_[a];
+        tN[pos] = n_[a];
+        tR[pos] = r_[a];
       }
-      lines[g * LP + (int)(jq / L) * P + (int)(jq % L)] = val;
-      jq += 256;
-      while (jq >= nv) { jq -= nv; ++g; }
+    }
+  }
+  const int nvi = (int)nv, tot = rows * nvi;
+  const float inv_nv = 1.0f / (float)nvi;
+  // (line, position) of flat tile index q, q < 256*MAXL
+  auto split = [&](int qq, int &gq, int &jq) {
+    gq = (int)((float)qq * inv_nv);
+    jq = qq - gq * nvi;
+    if (jq < 0) { --gq; jq += nvi; }
+    if (jq >= nvi) { ++gq; jq -= nvi; }
+  };
+  {
+    double val[MAXL];
+#pragma unroll
+    for (int a = 0; a < MAXL; ++a) {
+      const int qq = tid + 256 * a;
+      int gq, jq;
+      split(qq < tot ? qq : 0, gq, jq);
+      const int64_t i = i0 + gq;
+      double v = w[i * nv + jq];
+      if (MODE == VM_IMPL) {
+        auto xw = [&](int64_t x, int64_t y) { return w[x * nv + y]; };
+        v += o.dt * a01_dot(c, o, i, jq, xw);
+      }
+      val[a] = v;
+    }
+#pragma unroll
+    for (int a = 0; a < MAXL; ++a) {
+      const int qq = tid + 256 * a;
+      if (qq < tot) {
+        int gq, jq;
+        split(qq, gq, jq);
+        lines[gq * LP + pos_of(jq)] = val[a];
+      }
     }
   }
   __syncthreads();
@@ -614,7 +737,7 @@ var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict
     if (in && k == cnt - 1) dl = d[k];
   }
   // Bottom fold of the right-hand side: d[nv-1] += f d[nv-2].
-  const double prev_last = __shfl_up(dl, 1, TPL);
+  const double prev_last = from_prev_lane<TPL>(dl, ln);
   if (fact && sj + cnt == nv) {
 #pragma unroll
     for (int k = 0; k < MAXL; ++k)
@@ -631,16 +754,8 @@ var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict
       }
   }
   // Segmented inclusive prefix scan, then the carry entering each chunk.
-#pragma unroll
-  for (int dd = 1; dd < TPL; dd <<= 1) {
-    const double Mo = __shfl_up(M, dd, TPL), Eo = __shfl_up(E, dd, TPL);
-    if (ln >= dd) {
-      E = E + M * Eo;
-      M = M * Mo;
-    }
-  }
-  double carry = __shfl_up(E, 1, TPL);
-  if (ln == 0) carry = 0.0;
+  wave_prefix_scan<TPL>(M, E, ln);
+  const double carry = from_prev_lane<TPL>(E, ln);
   M = 1.0;
   E = 0.0;
   if (fact) {
@@ -660,16 +775,8 @@ var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict
       }
   }
   // Segmented inclusive suffix scan (maps of higher lanes applied first).
-#pragma unroll
-  for (int dd = 1; dd < TPL; dd <<= 1) {
-    const double Mo = __shfl_down(M, dd, TPL), Eo = __shfl_down(E, dd, TPL);
-    if (ln + dd < TPL) {
-      E = E + M * Eo;
-      M = M * Mo;
-    }
-  }
-  double after = __shfl_down(E, 1, TPL);
-  if (ln == TPL - 1) after = 0.0;
+  wave_suffix_scan<TPL>(M, E, ln);
+  const double after = from_next_lane<TPL>(E, ln);
   if (fact) {
     double x = after;
 #pragma unroll
@@ -689,14 +796,13 @@ var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict
   }
   __syncthreads();
   double *ob = out + o.off + i0 * nv;
-  {
-    const int64_t tot = (int64_t)rows * nv;
-    int64_t gq = 0, jq = tid;
-    while (jq >= nv) { jq -= nv; ++gq; }
-    for (int64_t qq = tid; qq < tot; qq += 256) {
-      ob[qq] = lines[gq * LP + (int)(jq / L) * P + (int)(jq % L)];
-      jq += 256;
-      while (jq >= nv) { jq -= nv; ++gq; }
+#pragma unroll
+  for (int a = 0; a < MAXL; ++a) {
+    const int qq = tid + 256 * a;
+    if (qq < tot) {
+      int gq, jq;
+      split(qq, gq, jq);
+      ob[qq] = lines[gq * LP + pos_of(jq)];
     }
   }
 }
@@ -754,9 +860,24 @@ __global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *ro
   col[SP_WSUP * sp_n] = ws[2];
   col[SP_S * sp_n] = S;
   col[SP_DIR * sp_n] = dir ? 1.0 : 0.0;
+  // AoS row of the spot sweep (layout at kSpRow): neighbour-of-Dirichlet band
+  // entries zeroed, cross weights premultiplied by S, [S != 0] flag.
   double *row = spr + g * 16;
-  for (int f = 0; f < SP_NF; ++f) row[f] = col[f * sp_n];
-  for (int f = SP_NF; f < 16; ++f) row[f] = 0.0;
+  const bool zsub = i == 1;                       // row 0 is Dirichlet
+  const bool zsup = o.barrier && i == n - 2;      // row n-1 is Dirichlet
+  row[0] = zsub ? 0.0 : col[SP_PSUB * sp_n];
+  row[1] = col[SP_PMID * sp_n];
+  row[2] = zsup ? 0.0 : col[SP_PSUP * sp_n];
+  row[3] = zsub ? 0.0 : col[SP_QSUB * sp_n];
+  row[4] = col[SP_QMID * sp_n];
+  row[5] = zsup ? 0.0 : col[SP_QSUP * sp_n];
+  row[6] = col[SP_PR * sp_n];
+  row[7] = col[SP_QR * sp_n];
+  row[8] = S * ws[0];
+  row[9] = S * ws[1];
+  row[10] = S * ws[2];
+  row[11] = S != 0.0 ? 1.0 : 0.0;
+  for (int f = 12; f < 16; ++f) row[f] = 0.0;
 }
 
 // One thread per variance node: A1 rows for S != 0 lines and cross weights.
@@ -1262,6 +1383,9 @@ int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale, 
                 "fdadi_hb_configure: unknown scheme");
   FDADI_REQUIRE(dt_scale > 0, FDADI_ERR_ARG, "fdadi_hb_configure: dt_scale must be positive");
   hipStream_t s = as_stream(stream);
+  // A captured graph stays valid while the scheme (the kernel sequence) is
+  // unchanged: theta and dt live in device memory and are re-read each step.
+  const bool keep_graph = scheme == hb->scheme;
   hb->scheme = scheme;
   hb->theta = theta;
   hb->dt_scale = dt_scale;
@@ -1282,13 +1406,15 @@ int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale, 
   FDADI_TRY_HIP(hipMemcpyAsync(hb->h_opts.data(), hb->d_opts, sizeof(HOpt) * hb->nopt,
                                hipMemcpyDeviceToHost, s));
   FDADI_TRY_HIP(hipStreamSynchronize(s));
-  if (hb->gexec) {
-    (void)hipGraphExecDestroy(hb->gexec);
-    hb->gexec = nullptr;
-  }
-  if (hb->graph) {
-    (void)hipGraphDestroy(hb->graph);
-    hb->graph = nullptr;
+  if (!keep_graph) {
+    if (hb->gexec) {
+      (void)hipGraphExecDestroy(hb->gexec);
+      hb->gexec = nullptr;
+    }
+    if (hb->graph) {
+      (void)hipGraphDestroy(hb->graph);
+      hb->graph = nullptr;
+    }
   }
   return FDADI_OK;
 }

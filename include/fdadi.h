@@ -187,7 +187,9 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
                     const double *vars, void *stream, fdadi_hb **out);
 int fdadi_hb_destroy(fdadi_hb *hb);
 /* Select the scheme, theta and a multiplier applied to every option's dt;
- * rebuilds the (time-invariant) factorisations on the device. */
+ * rebuilds the (time-invariant) factorisations on the device.  A graph
+ * captured by fdadi_hb_capture survives a configure that keeps the scheme
+ * (theta and dt are read from device memory at every step). */
 int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale,
                        void *stream);
 /* State of option k lives at d_state + offset[k] (layout (nS,nv) row-major). */

@@ -87,22 +87,30 @@ def test_training_on_adi_labels_beats_baselines(fd):
 
 def test_batched_labels_match_oracle(fd):
     """The batched engine's labels on the surrogate's label meshes equal the
-    oracle's Craig-Sneyd solve on the same meshes (32 random rows)."""
+    oracle's Craig-Sneyd solve on the same meshes (32 random rows), within the
+    grid criterion of tests/parity.py applied at the price node: the label
+    meshes are coarse and some parameter rows are nearly unstable, where the
+    oracle's own rounding noise (reordered summation) reaches 1e-6."""
     from fd_adi_amd.surrogate import adi_prices, label_grids, BOX_LO, BOX_HI
     from fd_adi_amd.utils import nearest_index
     from oracle.engine import HestonParams, HestonOracle
+    from tests.parity import grid_tolerance
     rng = np.random.default_rng(11)
     P = rng.uniform(BOX_LO, BOX_HI, (32, 8))
     nS, nv, nt = 48, 16, 40
     y = adi_prices(P, nS=nS, nv=nv, nt=nt)
     S, V = label_grids(P, nS, nv)
-    want = []
+    rels = []
     for k, r in enumerate(P):
         hp = HestonParams(spot=100.0, strike=100 * r[7], r=r[5], v0=r[4], T=r[6], kappa=r[0],
                           theta=r[1], sigma=r[2], rho=r[3])
         E = HestonOracle(hp, S[k], V[k])
-        out = E.solve_craigsneyd(nt, r[6] / nt, E.payoff(), 0.5)
-        want.append(out[nearest_index(S[k], 100.0), nearest_index(V[k], r[4])])
-    want = np.array(want)
-    ok = np.abs(want) < 1e3
-    npt.assert_allclose(y[ok], want[ok], rtol=1e-11, atol=1e-11)
+        Ea = HestonOracle(hp, S[k], V[k], firsts_reversed=True)
+        W = E.solve_craigsneyd(nt, r[6] / nt, E.payoff(), 0.5)
+        Wa = Ea.solve_craigsneyd(nt, r[6] / nt, Ea.payoff(), 0.5)
+        tol, floor = grid_tolerance(W, Wa)
+        want = W[nearest_index(S[k], 100.0), nearest_index(V[k], r[4])]
+        assert abs(y[k] - want) <= tol, (k, y[k], want, tol, floor)
+        rels.append(abs(y[k] - want) / abs(want))
+    print("label rel errors: median %.2e max %.2e" % (np.median(rels), np.max(rels)))
+    assert np.median(rels) < 1e-13
