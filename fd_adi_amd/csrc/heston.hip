@@ -496,7 +496,19 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   double2 ta0, tb0, ta1, tb1;  // table rows i and i+1
   quad_row_load(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
   quad_row_load(spr + clamp_row(s + 1) * kSpRow, jj, ta1, tb1);
-  double rd0 = rd[clamp_row(s) * unv + ujc], rd1 = rd[clamp_row(s + 1) * unv + ujc];
+  // Pivots: rd_i = 1 / (b_i - (a_i rd_{i-1}) c_{i-1}) down the chunk, from the
+  // stored pivot and super-diagonal of the row before the chunk (the same
+  // recurrence the factorisation ran).
+  double rdp = 0.0, cp = 0.0;
+  {
+    double2 tpa, tpb;
+    quad_row_load(spr + clamp_row(s - 1) * kSpRow, jj, tpa, tpb);
+    const SpotRow Rp = quad_row_fields(tpa, tpb);
+    if (s > 0) {
+      rdp = rd[clamp_row(s - 1) * unv + ujc];
+      cp = -tdt * (Rp.psup + q.v * Rp.qsup);
+    }
+  }
   double M = 1.0, E = 0.0;
 #pragma unroll
   for (int k = 0; k < MAXL; ++k) {
@@ -506,7 +518,6 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     double2 ta2, tb2;
     const unsigned i2 = clamp_row(i + 2), i3 = clamp_row(i + 3);
     quad_row_load(spr + i2 * kSpRow, jj, ta2, tb2);
-    const double rd2 = rd[i2 * unv + ujc];
     double nu[3], ny[3];
     load_row32(u, i3, unv, jm, ujc, jp, nu);
     if (HAS_Y) load_row32(y2, i3, unv, jm, ujc, jp, ny);
@@ -553,10 +564,14 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
       }
     }
     if (i == 0 || i == dir_hi) rhs = 0.0;  // the solve forces Dirichlet rows to 0
-    // I - tdt*A0 on this row (zero off-diagonals on Dirichlet rows and at the
-    // line ends come from the bands), scaled by the pivot.
+    // I - tdt*A0 on this row (identity rows on Dirichlet rows and zero
+    // off-diagonals at the line ends come from the bands), scaled by the pivot.
     const double aL = -tdt * (R.psub + q.v * R.qsub);
+    const double bL = 1.0 - tdt * (R.pmid + q.v * R.qmid);
     const double cL = -tdt * (R.psup + q.v * R.qsup);
+    const double rd0 = 1.0 / (bL - (aL * rdp) * cp);
+    rdp = rd0;
+    cp = cL;
     const double dsc = rd0 * rhs;
     mu[k] = -aL * rd0;
     nreg[k] = -cL * rd0;
@@ -568,7 +583,6 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     }
     // rotate the rings
     ta0 = ta1; tb0 = tb1; ta1 = ta2; tb1 = tb2;
-    rd0 = rd1; rd1 = rd2;
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       wu[0][b] = wu[1][b]; wu[1][b] = wu[2][b]; wu[2][b] = wu[3][b]; wu[3][b] = nu[b];
@@ -622,188 +636,160 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
 
 // ---- variance sweep -------------------------------------------------------------
 // Lines along the variance axis are contiguous.  A work-group of 256 threads
-// handles G = 256/TPL adjacent spot rows of one option.  The rows are loaded
-// with coalesced reads into LDS, in a chunked layout: position j of a line
-// sits at (j / L) * P + j % L, with L = ceil(nv / TPL) positions per chunk
-// and an odd chunk pitch P, so that the TPL lanes of a line reading chunk
-// position k hit distinct banks.  The factorisation tables (M, N, RD, shared
+// handles G = 256/TPL adjacent spot rows of one option; lane ln of a line owns
+// positions [ln*L, ln*L + L) with L = ceil(nv / TPL) <= MAXL.  The rows are
+// staged in LDS in a slot layout: line g, chunk ln, slot k (k < MAXL) at
+// g*LP + ln*(MAXL+1) + k (odd pitch: the TPL lanes reading slot k hit
+// distinct banks).  Slots past a chunk's L positions or past the line end
+// hold the identity of the two recurrences (right-hand side 0, multipliers
+// 1, pivot 0), so every chunk is MAXL slots long and the loops carry no
+// guards.  Staging walks the 256*MAXL slots of the tile with shifts only
+// (G * TPL * MAXL = 256 * MAXL).  The factorisation tables (M, N, RD, shared
 // by every S != 0 line of the option) are staged the same way.  Each lane
 // then solves its chunk in registers; the chunk carries of both triangular
-// sweeps are combined with a segmented wave scan of affine maps, the result
+// sweeps are combined with segmented wave scans of affine maps, the result
 // goes back to LDS and is stored with coalesced writes.  The S = 0 line is
 // diagonal.
 template <int MODE, int TPL, int MAXL>
 __global__ void __launch_bounds__(256)
 var_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ W,
                  double *__restrict__ out) {
+  static_assert((MAXL & (MAXL - 1)) == 0 && (TPL & (TPL - 1)) == 0, "powers of two");
   extern __shared__ double lds[];
   constexpr int G = 256 / TPL;
+  constexpr int P = MAXL + 1;         // slot pitch of a chunk
+  constexpr int LP = TPL * P + 1;     // line pitch
+  constexpr int SL = TPL * MAXL;      // slots per line
+  constexpr int NI = G * SL / 256;    // slots per thread (= MAXL)
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const HOpt o = c.opts[t.opt];
-  const int64_t nS = o.nS, nv = o.nv, i0 = t.start;
-  const int L = (int)((nv + TPL - 1) / TPL);
-  const int P = L | 1;
-  const int LP = TPL * P + 1;  // line pitch (odd)
-  const float inv_L = 1.0f / (float)L;
-  // LDS position of line position jq: jq + (P - L) * (jq / L)
-  auto pos_of = [&](int jq) {
-    int dq = (int)((float)jq * inv_L);
-    const int rq = jq - dq * L;
-    if (rq < 0) --dq;
-    if (rq >= L) ++dq;
-    return jq + (P - L) * dq;
-  };
+  const int nS = (int)o.nS, nv = (int)o.nv, i0 = t.start;
+  const int L = (nv + TPL - 1) / TPL;
   double *tM = lds, *tN = tM + LP, *tR = tN + LP, *lines = tR + LP;
   const int tid = threadIdx.x;
-  const int rows = (int)(nS - i0 < G ? nS - i0 : G);
+  const int rows = nS - i0 < G ? nS - i0 : G;
   const double *w = W + o.off;
-  // Stage the tables and the rows.  Loop bounds are compile-time (a line has
-  // at most TPL*MAXL positions, a tile at most 256*MAXL) so that every load
-  // of a thread is issued before the first LDS write.
-  constexpr int TQ = (TPL * MAXL + 255) / 256;
-  {
-    double m_[TQ], n_[TQ], r_[TQ];
+  const unsigned unv = (unsigned)nv;
+
+  // Stage the tables (one line of slots) ...
 #pragma unroll
-    for (int a = 0; a < TQ; ++a) {
-      const int jq = tid + 256 * a;
-      const int jl = jq < nv ? jq : 0;
-      m_[a] = VT(c, o, VT_M, jl);
-      n_[a] = VT(c, o, VT_N, jl);
-      r_[a] = VT(c, o, VT_RD, jl);
-    }
-#pragma unroll
-    for (int a = 0; a < TQ; ++a) {
-      const int jq = tid + 256 * a;
-      if (jq < nv) {
-        const int pos = pos_of(jq);
-        tM[pos] = m_[a];
-        tN[pos] = n_[a];
-        tR[pos] = r_[a];
-      }
+  for (int a = 0; a < (SL + 255) / 256; ++a) {
+    const int sq = tid + 256 * a;
+    if (SL >= 256 || sq < SL) {
+      const int ln = sq / MAXL, k = sq % MAXL;
+      const int j = ln * L + k;
+      const bool valid = k < L && j < nv;
+      const int jl = valid ? j : 0;
+      const double m = VT(c, o, VT_M, jl), n = VT(c, o, VT_N, jl), r = VT(c, o, VT_RD, jl);
+      const int slot = ln * P + k;
+      tM[slot] = valid ? m : 1.0;
+      tN[slot] = valid ? n : 1.0;
+      tR[slot] = valid ? r : 0.0;
     }
   }
-  const int nvi = (int)nv, tot = rows * nvi;
-  const float inv_nv = 1.0f / (float)nvi;
-  // (line, position) of flat tile index q, q < 256*MAXL
-  auto split = [&](int qq, int &gq, int &jq) {
-    gq = (int)((float)qq * inv_nv);
-    jq = qq - gq * nvi;
-    if (jq < 0) { --gq; jq += nvi; }
-    if (jq >= nvi) { ++gq; jq -= nvi; }
-  };
+  // ... and the rows (all loads first, then the LDS writes).
   {
-    double val[MAXL];
+    double val[NI];
 #pragma unroll
-    for (int a = 0; a < MAXL; ++a) {
-      const int qq = tid + 256 * a;
-      int gq, jq;
-      split(qq < tot ? qq : 0, gq, jq);
-      const int64_t i = i0 + gq;
-      double v = w[i * nv + jq];
+    for (int a = 0; a < NI; ++a) {
+      const int q = tid + 256 * a;
+      const int g = q / SL, sq = q % SL;
+      const int ln = sq / MAXL, k = sq % MAXL;
+      const int j = ln * L + k;
+      const bool valid = g < rows && k < L && j < nv;
+      const unsigned ii = (unsigned)(i0 + (g < rows ? g : 0));
+      const unsigned jl = valid ? (unsigned)j : 0u;
+      double v = w[ii * unv + jl];
       if (MODE == VM_IMPL) {
         auto xw = [&](int64_t x, int64_t y) { return w[x * nv + y]; };
-        v += o.dt * a01_dot(c, o, i, jq, xw);
+        v += o.dt * a01_dot(c, o, (int64_t)ii, (int64_t)jl, xw);
       }
-      val[a] = v;
+      val[a] = valid ? v : 0.0;
     }
 #pragma unroll
-    for (int a = 0; a < MAXL; ++a) {
-      const int qq = tid + 256 * a;
-      if (qq < tot) {
-        int gq, jq;
-        split(qq, gq, jq);
-        lines[gq * LP + pos_of(jq)] = val[a];
-      }
+    for (int a = 0; a < NI; ++a) {
+      const int q = tid + 256 * a;
+      const int g = q / SL, sq = q % SL;
+      const int ln = sq / MAXL, k = sq % MAXL;
+      lines[g * LP + ln * P + k] = val[a];
     }
   }
   __syncthreads();
   const int ln = tid % TPL, g = tid / TPL;
-  const int64_t i = i0 + g;
-  const int64_t sj = (int64_t)ln * L;
-  const int cnt = (int)(sj >= nv ? 0 : (sj + L <= nv ? L : nv - sj));
+  const int i = i0 + g;
   const bool row_ok = g < rows;
   const bool diag = row_ok && SP(c, o, SP_S, i) == 0.0;
-  const bool fact = row_ok && !diag && cnt > 0;
+  const bool fact = row_ok && !diag;
+  // Bottom fold of the right-hand side, d[nv-1] += f d[nv-2], by one lane
+  // per line (the slots of both positions are staged).
+  if (fact && ln == 0) {
+    const int j1 = nv - 1, j2 = nv - 2;
+    double *ln0 = lines + g * LP;
+    ln0[(j1 / L) * P + j1 % L] += o.fold_f * ln0[(j2 / L) * P + j2 % L];
+  }
+  __syncthreads();
   double *lp = lines + g * LP + ln * P;
   double d[MAXL], mreg[MAXL], nreg[MAXL], rreg[MAXL];
-  double dl = 0.0;  // this lane's last value (for the bottom fold)
 #pragma unroll
   for (int k = 0; k < MAXL; ++k) {
-    const bool in = row_ok && k < cnt;
-    d[k] = in ? lp[k] : 0.0;
-    mreg[k] = in ? tM[ln * P + k] : 0.0;
-    nreg[k] = in ? tN[ln * P + k] : 0.0;
-    rreg[k] = in ? tR[ln * P + k] : 0.0;
-    if (in && k == cnt - 1) dl = d[k];
-  }
-  // Bottom fold of the right-hand side: d[nv-1] += f d[nv-2].
-  const double prev_last = from_prev_lane<TPL>(dl, ln);
-  if (fact && sj + cnt == nv) {
-#pragma unroll
-    for (int k = 0; k < MAXL; ++k)
-      if (k == cnt - 1) d[k] += o.fold_f * (k > 0 ? d[k > 0 ? k - 1 : 0] : prev_last);
+    d[k] = lp[k];
+    mreg[k] = tM[ln * P + k];
+    nreg[k] = tN[ln * P + k];
+    rreg[k] = tR[ln * P + k];
   }
   // Local forward recurrence y = d + m y.
   double M = 1.0, E = 0.0;
-  if (fact) {
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k)
-      if (k < cnt) {
-        E = d[k] + mreg[k] * E;
-        M = mreg[k] * M;
-      }
+  for (int k = 0; k < MAXL; ++k) {
+    E = d[k] + mreg[k] * E;
+    M = mreg[k] * M;
   }
   // Segmented inclusive prefix scan, then the carry entering each chunk.
   wave_prefix_scan<TPL>(M, E, ln);
   const double carry = from_prev_lane<TPL>(E, ln);
-  M = 1.0;
-  E = 0.0;
-  if (fact) {
+  {
     double y = carry;
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k)
-      if (k < cnt) {
-        y = d[k] + mreg[k] * y;
-        d[k] = y;
-      }
-    // Local backward recurrence x = rd y + n x'.
+    for (int k = 0; k < MAXL; ++k) {
+      y = d[k] + mreg[k] * y;
+      d[k] = y;
+    }
+  }
+  // Local backward recurrence x = rd y + n x'.
+  M = 1.0;
+  E = 0.0;
 #pragma unroll
-    for (int k = MAXL - 1; k >= 0; --k)
-      if (k < cnt) {
-        E = rreg[k] * d[k] + nreg[k] * E;
-        M = nreg[k] * M;
-      }
+  for (int k = MAXL - 1; k >= 0; --k) {
+    E = rreg[k] * d[k] + nreg[k] * E;
+    M = nreg[k] * M;
   }
   // Segmented inclusive suffix scan (maps of higher lanes applied first).
   wave_suffix_scan<TPL>(M, E, ln);
   const double after = from_next_lane<TPL>(E, ln);
-  if (fact) {
+  {
     double x = after;
 #pragma unroll
-    for (int k = MAXL - 1; k >= 0; --k)
-      if (k < cnt) {
-        x = rreg[k] * d[k] + nreg[k] * x;
-        d[k] = x;
-      }
-  } else if (diag) {
-#pragma unroll
-    for (int k = 0; k < MAXL; ++k) d[k] *= o.rdS0;
-  }
-  if (row_ok) {
-#pragma unroll
-    for (int k = 0; k < MAXL; ++k)
-      if (k < cnt) lp[k] = d[k];
-  }
-  __syncthreads();
-  double *ob = out + o.off + i0 * nv;
-#pragma unroll
-  for (int a = 0; a < MAXL; ++a) {
-    const int qq = tid + 256 * a;
-    if (qq < tot) {
-      int gq, jq;
-      split(qq, gq, jq);
-      ob[qq] = lines[gq * LP + pos_of(jq)];
+    for (int k = MAXL - 1; k >= 0; --k) {
+      x = rreg[k] * d[k] + nreg[k] * x;
+      d[k] = x;
     }
+  }
+  if (diag) {
+    // the S = 0 line is diagonal: x = rdS0 * rhs (slots, not the solve above)
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) d[k] = o.rdS0 * lp[k];
+  }
+#pragma unroll
+  for (int k = 0; k < MAXL; ++k) lp[k] = d[k];
+  __syncthreads();
+  double *ob = out + o.off + (int64_t)i0 * nv;
+#pragma unroll
+  for (int a = 0; a < NI; ++a) {
+    const int q = tid + 256 * a;
+    const int gq = q / SL, sq = q % SL;
+    const int lq = sq / MAXL, k = sq % MAXL;
+    const int j = lq * L + k;
+    if (gq < rows && k < L && j < nv) ob[(unsigned)gq * unv + (unsigned)j] = lines[gq * LP + lq * P + k];
   }
 }
 
@@ -1158,9 +1144,8 @@ int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *ou
 
 template <int MODE, int TPL>
 int launch_var_t(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
-  // LDS: three tables and 256/TPL lines, each TPL chunks of odd pitch + 1.
-  const int64_t L = (hb->nvmax + TPL - 1) / TPL;
-  const size_t lds = sizeof(double) * (3 + 256 / TPL) * (TPL * (L | 1) + 1);
+  // LDS: three tables and 256/TPL lines of TPL chunks (pitch kMAXL + 1) + 1.
+  const size_t lds = sizeof(double) * (3 + 256 / TPL) * (TPL * (kMAXL + 1) + 1);
   FDADI_RET(allow_lds((const void *)var_sweep_kernel<MODE, TPL, kMAXL>, lds));
   hipLaunchKernelGGL((var_sweep_kernel<MODE, TPL, kMAXL>), dim3(hb->nvtiles), dim3(256), lds, s,
                      ctx_of(hb), hb->d_vtiles, W, out);
