@@ -411,15 +411,18 @@ __device__ __forceinline__ SpotRow quad_row_fields(double2 a, double2 b) {
   return r;
 }
 
-// Row rr (already clamped to the grid) of a field at columns jm, j, jp.
-// Offsets are 32-bit unsigned (an option's grid has at most 2048 x 1024
-// nodes), so the loads address from a scalar base.
+// Row rr (already clamped to the grid) of a field at columns jm, j, jp, and
+// at column jm2 = j - 2 on the last column only (the second-order backward
+// row of d/dv reaches two columns back; elsewhere out[3] = 0).  Offsets are
+// 32-bit unsigned (an option's grid has at most 2048 x 1024 nodes).
 __device__ __forceinline__ void load_row32(const double *__restrict__ f, unsigned rr, unsigned nv,
-                                           unsigned jm, unsigned j, unsigned jp, double out[3]) {
+                                           unsigned jm, unsigned j, unsigned jp, unsigned jm2,
+                                           bool last, double out[4]) {
   const unsigned b = rr * nv;
   out[0] = f[b + jm];
   out[1] = f[b + j];
   out[2] = f[b + jp];
+  out[3] = last ? f[b + jm2] : 0.0;
 }
 
 // A spot sweep: one work-group per (option, NL = 4 adjacent variance
@@ -486,12 +489,15 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   // Phase 1: walk the chunk with 3x3 windows, form the fused right-hand side
   // (and the next stage's explicit term), and run the local forward recurrence.
   double mu[MAXL], nreg[MAXL];
-  double wu[4][3], wy[4][3];  // rows i-1, i, i+1 and i+2 (in flight)
+  // rows i-1, i, i+1 and i+2 (in flight); [3] = column j-2 on the last column
+  double wu[4][4], wy[4][4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) load_row32(u, clamp_row(s - 1 + r), unv, jm, ujc, jp, wu[r]);
+  for (int r = 0; r < 4; ++r)
+    load_row32(u, clamp_row(s - 1 + r), unv, jm, ujc, jp, jm2, last, wu[r]);
   if (HAS_Y) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) load_row32(y2, clamp_row(s - 1 + r), unv, jm, ujc, jp, wy[r]);
+    for (int r = 0; r < 4; ++r)
+      load_row32(y2, clamp_row(s - 1 + r), unv, jm, ujc, jp, jm2, last, wy[r]);
   }
   double2 ta0, tb0, ta1, tb1;  // table rows i and i+1
   quad_row_load(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
@@ -513,16 +519,15 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
 #pragma unroll
   for (int k = 0; k < MAXL; ++k) {
     const int i = s + k;
-    const unsigned ic = clamp_row(i);
     // loads for two rows ahead
     double2 ta2, tb2;
     const unsigned i2 = clamp_row(i + 2), i3 = clamp_row(i + 3);
     quad_row_load(spr + i2 * kSpRow, jj, ta2, tb2);
-    double nu[3], ny[3];
-    load_row32(u, i3, unv, jm, ujc, jp, nu);
-    if (HAS_Y) load_row32(y2, i3, unv, jm, ujc, jp, ny);
-    const double uj2 = u[ic * unv + jm2];
-    const double yj2 = HAS_Y ? y2[ic * unv + jm2] : 0.0;
+    double nu[4], ny[4];
+    load_row32(u, i3, unv, jm, ujc, jp, jm2, last, nu);
+    if (HAS_Y) load_row32(y2, i3, unv, jm, ujc, jp, jm2, last, ny);
+    const double uj2 = wu[1][3];
+    const double yj2 = HAS_Y ? wy[1][3] : 0.0;
 
     const SpotRow R = quad_row_fields(ta0, tb0);
     Win wU, wY;
@@ -584,7 +589,7 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     // rotate the rings
     ta0 = ta1; tb0 = tb1; ta1 = ta2; tb1 = tb2;
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
+    for (int b = 0; b < 4; ++b) {
       wu[0][b] = wu[1][b]; wu[1][b] = wu[2][b]; wu[2][b] = wu[3][b]; wu[3][b] = nu[b];
       if (HAS_Y) {
         wy[0][b] = wy[1][b]; wy[1][b] = wy[2][b]; wy[2][b] = wy[3][b]; wy[3][b] = ny[b];
