@@ -128,6 +128,94 @@ def batched(n_opt, seed, K, W):
     return n_opt * K / wall, wall / K, npts
 
 
+def surrogate_run(n_labels, n_queries, members, steps, seed=17):
+    """Configs 3-4 at reduced scale: FP64 ADI labels from the batched engine
+    (256 x 128, nt = 200, Craig-Sneyd), an ensemble of MLPs trained on the
+    GPU, the spread gate calibrated on held-out labels for 1e-4 absolute
+    error, then serving of fresh queries (5% outside the training box) with
+    the ADI fallback; errors against FP64 ADI on a sample of the queries."""
+    import numpy as np
+    import fd_adi_amd
+    from fd_adi_amd import surrogate as S
+    out = {"labels": n_labels, "queries": n_queries, "members": members,
+           "train_steps": steps, "batch": 8192, "grid": "256x128 nt=200 cs"}
+    t = time.perf_counter()
+    P = S.draw_params(n_labels, seed)
+    y = S.adi_prices(P)
+    fd_adi_amd._lib.sync()
+    t_lab = time.perf_counter() - t
+    ok = S.within_call_bounds(P, y)
+    out["label_options_per_s"] = round(n_labels / t_lab, 1)
+    out["labels_dropped_unstable"] = int((~ok).sum())
+    P, y = P[ok], y[ok]
+    nval = max(len(y) // 10, 1)
+    Pv, yv, Pt, yt = P[:nval], y[:nval], P[nval:], y[nval:]
+    t = time.perf_counter()
+    ens, hists = S.train_ensemble(Pt, yt, members=members, steps=steps, log_every=steps // 4)
+    fd_adi_amd._lib.sync()
+    t_train = time.perf_counter() - t
+    flop = 4.75e6 * 8192 * steps * members
+    out["train_sec"] = round(t_train, 2)
+    out["train_TFLOPs"] = round(flop / t_train / 1e12, 1)
+    out["train_final_loss"] = [round(h[-1][1], 6) for h in hists]
+    mean_v, spread_v = ens.predict(Pv)
+    out["val_rmse"] = float(np.sqrt(np.mean((mean_v - yv) ** 2)))
+    out["val_max_abs_err"] = float(np.max(np.abs(mean_v - yv)))
+    tau, acc = S.calibrate_gate(ens, Pv, yv, 1e-4)
+    out["gate_tau"] = tau
+    out["gate_val_accept"] = round(acc, 4)
+    # the same gate calibrated for looser tolerances (not used for serving)
+    out["val_accept_at_atol"] = {"%g" % a: round(S.calibrate_gate(ens, Pv, yv, a)[1], 4)
+                                 for a in (1e-3, 1e-2, 1e-1)}
+    pricer = S.SurrogatePricer(ens, tau)
+    Q = S.draw_params(n_queries, seed + 1, outside_frac=0.05)
+    t = time.perf_counter()
+    price, okq = pricer.price(Q)
+    fd_adi_amd._lib.sync()
+    t_serve = time.perf_counter() - t
+    out["options_per_s"] = round(n_queries / t_serve, 1)
+    out["fallback_rate"] = round(1.0 - okq.mean(), 4)
+    t = time.perf_counter()
+    _ = ens.predict(Q)
+    fd_adi_amd._lib.sync()
+    out["surrogate_only_options_per_s"] = round(n_queries / (time.perf_counter() - t), 1)
+    rng = np.random.default_rng(seed + 2)
+    idx = rng.choice(n_queries, min(1024, n_queries), replace=False)
+    truth = S.adi_prices(Q[idx])
+    err = np.abs(price[idx] - truth)
+    out["err_sample"] = len(idx)
+    out["max_abs_err"] = float(np.max(err))
+    out["mean_abs_err"] = float(np.mean(err))
+    return out
+
+
+def single_loops(n_op, n_fused, seed=2024):
+    """Config-2 options solved one at a time (host payoff in, host grid out):
+    the operator-algebra path (per-step BandedOperator calls, the reference's
+    GPU loop) and the fused one-option engine; options/s for each."""
+    from fd_adi_amd.workloads import batch_draws
+    from fd_adi_amd.FiniteDifferenceEngineGPU import HestonFiniteDifferenceEngine
+    from fd_adi_amd.Option import HestonOption
+    opts = batch_draws(max(n_op, n_fused) + 1, seed)
+    out = {}
+    for fused, n in ((False, n_op), (True, n_fused)):
+        runs = []
+        for o in opts[:n + 1]:
+            opt = HestonOption(spot=o.S0, strike=o.K, interest_rate=o.r, variance=o.v0,
+                               tenor=o.T, mean_reversion=o.kappa, mean_variance=o.theta,
+                               vol_of_variance=o.sigma, correlation=o.rho)
+            E = HestonFiniteDifferenceEngine(opt, spots=o.spots, vars=o.vars)
+            E.fused = fused
+            runs.append((E, o))
+        E, o = runs[0]
+        E.solve_craigsneyd(o.nt, o.dt, o.payoff(), 0.5)  # warm-up (kernel attributes)
+        t = time.perf_counter()
+        for E, o in runs[1:]:
+            E.solve_craigsneyd(o.nt, o.dt, o.payoff(), 0.5)
+        out["op_path" if not fused else "fused"] = round(n / (time.perf_counter() - t), 2)
+    return out
+
+
 def cpu_baseline(steps):
     """Oracle (NumPy/SciPy restatement of the reference CPU engine), one
     thread, ``steps`` HV steps of the 2048x1024 option; extrapolated to the
@@ -196,6 +284,10 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--surrogate-labels", type=int, default=1 << 15,
+                    help="ADI-labelled training draws for the surrogate (0 = skip)")
+    ap.add_argument("--surrogate-queries", type=int, default=1 << 15)
+    ap.add_argument("--surrogate-steps", type=int, default=1500)
     ap.add_argument("--probe", action="store_true",
                     help="profiling probe for rocprofv3 counter runs: 20 HV steps of the "
                          "2048x1024 option, then 5 launches of the scale kernel on 2^25 "
@@ -251,11 +343,20 @@ def main():
         w0, _, p0, _ = run_single(o0, "douglas", 0.5, 20, 2, smoothing=2)
         extra["plain_150x150_douglas_smoothed_options_per_s"] = round(20 / w0, 2)
         extra["plain_150x150_price"] = p0
+        loops = single_loops(2, 16)
+        extra["single_option_loop_cs_options_per_s"] = loops
         rate, sec, npts = batched(4096, 2024, 2, 1)
         extra["batched_4096_cs_options_per_s"] = round(rate, 1)
         extra["batched_4096_cs_sec"] = round(sec, 4)
         extra["batched_4096_cs_GBs_algorithmic"] = round(72 * npts * 200 / sec / 1e9, 1)
+        extra["batched_over_op_path_loop"] = round(rate / loops["op_path"], 1)
+        extra["batched_over_fused_loop"] = round(rate / loops["fused"], 1)
         log("extra: %s" % extra)
+    if args.surrogate_labels:
+        sur = surrogate_run(args.surrogate_labels, args.surrogate_queries, 4,
+                            args.surrogate_steps)
+        extra["surrogate"] = sur
+        log("surrogate: %s" % sur)
 
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "options/s", "n_gpus": 1,

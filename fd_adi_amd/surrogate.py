@@ -62,6 +62,11 @@ class DeviceInts(object):
     def p(self):
         return self.buf.p
 
+    def at(self, k):
+        """Device pointer to element k."""
+        base = self.buf.p.value if hasattr(self.buf.p, "value") else int(self.buf.p)
+        return ctypes.c_void_p(base + 4 * int(k))
+
 
 class MLP(object):
 
@@ -108,9 +113,11 @@ class MLP(object):
                   int(rows), ctypes.byref(loss), self.stream)
         return loss.value
 
-    def train_step(self, P, T, idx, rows, lr):
-        _lib.call("fdadi_mlp_train_step", self._h, P.p, T.p, idx.p if idx is not None else None,
-                  int(rows), ctypes.c_float(lr), self.stream)
+    def train_step(self, P, T, idx, rows, lr, start=0):
+        """One Adam step on rows idx[start:start+rows] (all rows if idx is None)."""
+        ip = None if idx is None else (idx.at(start) if start else idx.p)
+        _lib.call("fdadi_mlp_train_step", self._h, P.p, T.p, ip, int(rows), ctypes.c_float(lr),
+                  self.stream)
 
     def last_loss(self):
         loss = ctypes.c_float()
@@ -135,28 +142,38 @@ class MLP(object):
         T = SizedArrayPtr(np.ascontiguousarray(T_host, dtype=float))
         n = T_host.shape[0]
         rng = np.random.default_rng(seed)
-        perm, pos, hist = rng.permutation(n), 0, []
+        rows = min(batch, n)
+        per_epoch = max(n // rows, 1)
+        # one device permutation per epoch; minibatches are slices of it
+        perm, pos, hist = DeviceInts(rng.permutation(n)), 0, []
         for k in range(steps):
-            if pos + batch > n:
-                perm, pos = rng.permutation(n), 0
-            idx = DeviceInts(perm[pos:pos + batch])
-            pos += batch
-            self.train_step(P, T, idx, min(batch, n), lr)
+            if pos == per_epoch:
+                perm, pos = DeviceInts(rng.permutation(n)), 0
+            self.train_step(P, T, perm, rows, lr, start=pos * rows)
+            pos += 1
             if log_every and (k % log_every == 0 or k == steps - 1):
                 hist.append((k, self.last_loss()))
         return hist
 
 
+def lower_bound(P):
+    """European call lower bound max(S0 - K exp(-rT), 0) with S0 = 100: the
+    networks learn the price above it (a smaller, smoother target)."""
+    P = np.asarray(P, dtype=float)
+    return np.maximum(100.0 - 100.0 * P[:, 7] * np.exp(-P[:, 5] * P[:, 6]), 0.0)
+
+
 class Ensemble(object):
     """Independently initialised networks; the spread of their predictions
-    is the per-query error estimate."""
+    is the per-query error estimate.  Prices are lower_bound + network."""
 
     def __init__(self, members):
         self.members = list(members)
 
     def predict(self, P_host):
         Y = np.stack([m.predict(P_host) for m in self.members])
-        return Y.mean(axis=0), Y.std(axis=0, ddof=1) if len(self.members) > 1 else 0 * Y[0]
+        spread = Y.std(axis=0, ddof=1) if len(self.members) > 1 else 0 * Y[0]
+        return lower_bound(P_host) + Y.mean(axis=0), spread
 
 
 def label_grids(P, nS=256, nv=128):
@@ -211,6 +228,65 @@ def within_call_bounds(P, y, slack=1.0):
     y = np.asarray(y, dtype=float)
     lower = np.maximum(100.0 - 100.0 * P[:, 7] * np.exp(-P[:, 5] * P[:, 6]), 0.0)
     return np.isfinite(y) & (y >= lower - slack) & (y <= 100.0 + slack)
+
+
+def draw_params(n, seed, outside_frac=0.0, widen=0.25):
+    """``n`` feature rows drawn uniformly over the training box; a fraction
+    ``outside_frac`` of them instead lies outside it: drawn from the box
+    widened by ``widen`` of each range on both sides until at least one
+    feature is outside, then clipped to physically valid values (positive
+    variances, vol-of-vol, mean reversion and maturity; |rho| < 1)."""
+    rng = np.random.default_rng(seed)
+    P = rng.uniform(BOX_LO, BOX_HI, (n, N_IN))
+    n_out = int(round(outside_frac * n))
+    if n_out:
+        w = BOX_HI - BOX_LO
+        lo, hi = BOX_LO - widen * w, BOX_HI + widen * w
+        rows = []
+        while len(rows) < n_out:
+            x = rng.uniform(lo, hi, (2 * n_out, N_IN))
+            x = x[np.any((x < BOX_LO) | (x > BOX_HI), axis=1)]
+            rows.extend(list(x))
+        Q = np.array(rows[:n_out])
+        Q[:, 0] = np.maximum(Q[:, 0], 0.1)     # kappa
+        Q[:, 1] = np.maximum(Q[:, 1], 0.005)   # theta
+        Q[:, 2] = np.maximum(Q[:, 2], 0.05)    # sigma
+        Q[:, 3] = np.clip(Q[:, 3], -0.99, 0.99)
+        Q[:, 4] = np.maximum(Q[:, 4], 0.005)   # v0
+        Q[:, 6] = np.maximum(Q[:, 6], 0.05)    # T
+        Q[:, 7] = np.maximum(Q[:, 7], 0.3)     # K / S0
+        P[rng.choice(n, n_out, replace=False)] = Q
+    return P
+
+
+def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log_every=0):
+    """Independently initialised networks trained on the same labels (each on
+    its own minibatch order); returns the Ensemble and the loss histories."""
+    t = np.asarray(y, dtype=float) - lower_bound(P)
+    mu, sd = float(np.mean(t)), float(np.std(t))
+    nets, hists = [], []
+    for k in range(members):
+        m = MLP(BOX_LO, BOX_HI, mu, sd, seed=seed + 101 * k, max_rows=batch)
+        hists.append(m.fit(P, t, steps, batch=batch, lr=lr, seed=seed + 7 * k,
+                           log_every=log_every))
+        nets.append(m)
+    return Ensemble(nets), hists
+
+
+def calibrate_gate(ensemble, P, y, atol=1e-4):
+    """Largest spread threshold tau such that every validation query with
+    ensemble spread <= tau has |mean - y| <= atol (0 if none qualifies);
+    returns (tau, accepted fraction of the validation set)."""
+    mean, spread = ensemble.predict(P)
+    err = np.abs(mean - y)
+    order = np.argsort(spread)
+    bad = np.cumsum(err[order] > atol)
+    ok = np.where(bad == 0)[0]
+    if ok.size == 0:
+        return 0.0, 0.0
+    k = int(ok[-1])
+    # stop below the next spread value, which would admit a failing query
+    return float(spread[order[k]]), (k + 1) / float(len(y))
 
 
 class SurrogatePricer(object):
