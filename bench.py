@@ -112,6 +112,8 @@ def batched(n_opt, seed, K, W):
     init = hb.state_view().copy(True)
     view = hb.state_view()
     hb.configure("cs", 0.5, 1.0)
+    shapes = hb.autotune()
+    log("batch: autotuned launch shapes %s" % shapes)
     hb.step(2)
     hb.capture(200)
     for _ in range(W):

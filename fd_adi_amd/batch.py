@@ -76,6 +76,37 @@ class HestonBatch(object):
     def set_shape(self, kind, threads):
         _lib.call("fdadi_hb_set_shape", self._h, int(kind), int(threads), 0)
 
+    # launch-shape candidates per sweep kind (spot: threads per work-group;
+    # variance: lanes per line); set_shape rejects those too small for the grids
+    SHAPES = {0: (64, 128, 256, 512, 1024), 1: (1, 2, 4, 8, 16, 32, 64)}
+
+    def autotune(self, steps=2):
+        """Pick the fastest launch shape of each sweep kind for the configured
+        scheme: every valid candidate is timed over ``steps`` steps with HIP
+        events (the stage slots of that kind summed); the state is saved and
+        restored.  Returns {kind: threads} as set."""
+        if self.scheme is None:
+            raise RuntimeError("autotune: configure the engine first")
+        # stage slots of each kind in a step (fdadi_hb_step's launch order)
+        order = {0: (1, 0), 1: (0, 1)}.get(self.scheme, (0, 1, 0, 1))
+        saved = self.state_view().copy(True)
+        best = {}
+        for kind in (0, 1):
+            timings = []
+            for c in self.SHAPES[kind]:
+                try:
+                    self.set_shape(kind, c)
+                except (ValueError, NotImplementedError):
+                    continue
+                self.step_timed(1)
+                st = self.step_timed(steps)
+                timings.append((sum(t for t, k in zip(st, order) if k == kind), c))
+            best[kind] = min(timings)[1]
+            self.set_shape(kind, best[kind])
+        self.state_view().copy_from(saved)
+        self.sync()
+        return best
+
     def get_shape(self, kind):
         t, l = ctypes.c_int(), ctypes.c_int()
         _lib.call("fdadi_hb_get_shape", self._h, int(kind), ctypes.byref(t), ctypes.byref(l))

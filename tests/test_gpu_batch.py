@@ -146,3 +146,23 @@ def test_reference_instability_reproduced(fd):
     assert iv == iw
     npt.assert_allclose(np.abs(V).max(), np.abs(W).max(), rtol=1e-4)
     npt.assert_allclose(V[o.idx], W[o.idx], rtol=1e-10)
+
+
+def test_autotune_restores_state_and_keeps_results(fd):
+    P, S, Vs, dts, O = draw(5, 3, nS_range=(40, 120), nv_range=(20, 80))
+    init = [o.payoff() for o in O]
+    ref = fd.HestonBatch(P, S, Vs, dts)
+    ref.set_state(init)
+    ref.configure("cs", 0.5)
+    ref.step(6)
+    tuned = fd.HestonBatch(P, S, Vs, dts)
+    tuned.set_state(init)
+    tuned.configure("cs", 0.5)
+    before = tuned.get_packed()
+    shapes = tuned.autotune(steps=1)
+    assert set(shapes) == {0, 1}
+    npt.assert_array_equal(tuned.get_packed(), before)
+    assert tuned.get_shape(0)[0] == shapes[0] and tuned.get_shape(1)[0] == shapes[1]
+    tuned.step(6)
+    scale = np.max(np.abs(ref.get_packed()))
+    assert np.max(np.abs(tuned.get_packed() - ref.get_packed())) <= 1e-12 * scale
