@@ -1,0 +1,35 @@
+"""The price_one_option command line (argument handling on the CPU; one
+small solve on the GPU)."""
+
+import os
+
+import numpy as np
+import pytest
+
+from fd_adi_amd import price_one_option as cli
+
+
+def test_arguments_and_defaults():
+    opt = cli.read_args(["-nx", "40", "20", "-nt", "10", "--gpu"])
+    assert opt.nx == [40, 20] and opt.nt == 10 and opt.scheme == "hv"
+    assert opt.mean_variance == opt.variance == 0.04 and opt.strike == 99.0
+    with pytest.raises(SystemExit):
+        cli.read_args(["-nx", "40", "20", "-nt", "10", "--cpu"])
+    with pytest.raises(SystemExit):
+        cli.read_args(["-nx", "40", "20", "-nt", "10", "--top", "150"])
+
+
+@pytest.mark.gpu
+def test_cli_matches_engine(gpu, tmp_path, capsys):
+    args = ["-nx", "40", "20", "-nt", "10", "--gpu", "--scheme", "d", "-p", "-0.3",
+            "-o", "0.3", "--out-dir", str(tmp_path)]
+    assert cli.main(args) == 0
+    out = capsys.readouterr().out
+    price = float(out.split("FD:")[1].split()[0])
+    opt = cli.read_args(args)
+    e, p2 = cli.run(opt)
+    assert price == p2
+    files = os.listdir(tmp_path)
+    assert len(files) == 1 and files[0].startswith("gpu_moneyness_")
+    z = np.load(os.path.join(tmp_path, files[0]))
+    assert z["V"].shape == (40, 20) and z["V"][e.idx] == price
