@@ -88,36 +88,19 @@ __device__ __forceinline__ double VT(const Ctx &c, const HOpt &o, int f, int64_t
   return c.vt[f * c.vt_n + o.vo + j];
 }
 
-// (A0 x)(i,j) without residual; 0 on Dirichlet rows (their value is 0 and
-// the Dirichlet row of dt*A0 / theta*dt*A0 is the identity row).  Neighbour
-// values on Dirichlet rows are forced to 0 as apply does.
-template <class F>
-__device__ double a0_dot(const Ctx &c, const HOpt &o, int64_t i, int64_t j, double v, F x) {
-  if (SP(c, o, SP_DIR, i) != 0.0) return 0.0;
-  const double xm = (i > 0 && SP(c, o, SP_DIR, i - 1) == 0.0) ? x(i - 1, j) : 0.0;
-  const double xp = (i < o.nS - 1 && SP(c, o, SP_DIR, i + 1) == 0.0) ? x(i + 1, j) : 0.0;
-  const double sub = SP(c, o, SP_PSUB, i) + v * SP(c, o, SP_QSUB, i);
-  const double mid = SP(c, o, SP_PMID, i) + v * SP(c, o, SP_QMID, i);
-  const double sup = SP(c, o, SP_PSUP, i) + v * SP(c, o, SP_QSUP, i);
-  return sub * xm + mid * x(i, j) + sup * xp;
+// A0 band entry p + w2 * c00 as the reference's combined operator holds it:
+// the (0,) band (with -r/2 on the diagonal) plus the (0,0) weight scaled by
+// c00 = (1/2 v) S^2, one rounding each (no fused multiply-add), so that the
+// discrete operator is bit-identical to the reference's.
+__device__ __forceinline__ double band_add(double p, double w2, double c00) {
+#pragma clang fp contract(off)
+  const double t = w2 * c00;
+  return p + t;
 }
 
-__device__ __forceinline__ double r0_at(const Ctx &c, const HOpt &o, int64_t i, double v) {
-  return SP(c, o, SP_PR, i) + v * SP(c, o, SP_QR, i);
-}
-
-// (A1 x)(i,j), including the -2 band entry of the last variance row.
-template <class F>
-__device__ double a1_dot(const Ctx &c, const HOpt &o, int64_t i, int64_t j, F x) {
-  const double snz = SP(c, o, SP_S, i) != 0.0 ? 1.0 : 0.0;
-  const double mid = snz * VT(c, o, VT_GMID, j) - 0.5 * o.r;
-  double acc = mid * x(i, j);
-  if (snz != 0.0) {
-    if (j > 0) acc += VT(c, o, VT_GSUB, j) * x(i, j - 1);
-    if (j < o.nv - 1) acc += VT(c, o, VT_GSUP, j) * x(i, j + 1);
-    if (j == o.nv - 1) acc += o.Gz * x(i, j - 2);
-  }
-  return acc;
+__device__ __forceinline__ double c00_of(double v, double S) {
+#pragma clang fp contract(off)
+  return (0.5 * v) * (S * S);
 }
 
 // (A01 x)(i,j): rho sigma S v times the 9-point central product stencil.
@@ -136,48 +119,6 @@ __device__ double a01_dot(const Ctx &c, const HOpt &o, int64_t i, int64_t j, F x
   return coef * acc;
 }
 
-// Right-hand side of a spot-sweep stage at (i, j).
-template <int MODE>
-__device__ double spot_rhs(const Ctx &c, const HOpt &o, int64_t i, int64_t j,
-                           const double *__restrict__ U, const double *__restrict__ Y) {
-  const double *u = U + o.off;
-  const int64_t nv = o.nv;
-  auto xu = [&](int64_t a, int64_t b) { return u[a * nv + b]; };
-  const double v = VT(c, o, VT_V, j);
-  if (MODE == SM_IMPL) return xu(i, j) + o.dt * r0_at(c, o, i, v);
-  const double dt = o.dt;
-  const double a0u = a0_dot(c, o, i, j, v, xu);
-  const double r0 = SP(c, o, SP_DIR, i) != 0.0 ? 0.0 : r0_at(c, o, i, v);
-  const double fu = dt * (a0u + r0) + dt * a1_dot(c, o, i, j, xu) + dt * a01_dot(c, o, i, j, xu);
-  double rhs = xu(i, j) + fu;
-  if (MODE == SM_FIRST) return rhs - o.tdt * a0u;
-  const double *y = Y + o.off;
-  auto xy = [&](int64_t a, int64_t b) { return y[a * nv + b]; };
-  auto xd = [&](int64_t a, int64_t b) { return y[a * nv + b] - u[a * nv + b]; };
-  if (MODE == SM_HV) {
-    const double fd = dt * a0_dot(c, o, i, j, v, xd) + dt * a1_dot(c, o, i, j, xd) +
-                      dt * a01_dot(c, o, i, j, xd);
-    return rhs + 0.5 * fd - o.tdt * a0_dot(c, o, i, j, v, xy);
-  }
-  if (MODE == SM_CS) return rhs + (0.5 * dt) * a01_dot(c, o, i, j, xd) - o.tdt * a0u;
-  // SM_MCS: theta = tdt / dt
-  const double th = o.tdt / dt;
-  const double c01 = a01_dot(c, o, i, j, xd);
-  const double fd = dt * (a0_dot(c, o, i, j, v, xd) + r0) + dt * a1_dot(c, o, i, j, xd) + dt * c01;
-  return rhs + (th * dt) * c01 + (0.5 - th) * fd - o.tdt * a0u;
-}
-
-// Output of a spot-sweep stage at (i, j) given the solved value x.
-template <int MODE>
-__device__ double spot_out(const Ctx &c, const HOpt &o, int64_t i, int64_t j, double x,
-                           const double *__restrict__ U, const double *__restrict__ Y) {
-  if (MODE == SM_IMPL) return x;
-  const double *f = (MODE == SM_HV ? Y : U) + o.off;
-  const int64_t nv = o.nv;
-  auto xf = [&](int64_t a, int64_t b) { return f[a * nv + b]; };
-  return x - o.tdt * a1_dot(c, o, i, j, xf);
-}
-
 // Coefficients of the tridiagonal I - tdt*A0 on spot line j at row i.
 __device__ __forceinline__ void li0_row(const Ctx &c, const HOpt &o, int64_t i, double v,
                                         double &a, double &b, double &cc) {
@@ -185,9 +126,10 @@ __device__ __forceinline__ void li0_row(const Ctx &c, const HOpt &o, int64_t i, 
     a = 0.0; b = 1.0; cc = 0.0;
     return;
   }
-  a = -o.tdt * (SP(c, o, SP_PSUB, i) + v * SP(c, o, SP_QSUB, i));
-  b = 1.0 - o.tdt * (SP(c, o, SP_PMID, i) + v * SP(c, o, SP_QMID, i));
-  cc = -o.tdt * (SP(c, o, SP_PSUP, i) + v * SP(c, o, SP_QSUP, i));
+  const double c00 = c00_of(v, SP(c, o, SP_S, i));
+  a = -o.tdt * band_add(SP(c, o, SP_PSUB, i), SP(c, o, SP_QSUB, i), c00);
+  b = 1.0 - o.tdt * band_add(SP(c, o, SP_PMID, i), SP(c, o, SP_QMID, i), c00);
+  cc = -o.tdt * band_add(SP(c, o, SP_PSUP, i), SP(c, o, SP_QSUP, i), c00);
   if (i == 0) a = 0.0;
   if (i == o.nS - 1) cc = 0.0;
 }
@@ -327,8 +269,11 @@ __device__ void affine_scan(const double *cM, const double *cE, double *cIn, int
 // ---- spot sweep ---------------------------------------------------------------
 // One 128-byte row of per-node spot coefficients, built from the spot table
 // with the boundary handling folded in:
-// [psub' pmid psup' qsub' | qmid qsup' pr qr | S*wsub S*wmid S*wsup snz | 0 0 0 0]
-// where ' marks band entries zeroed when they multiply a Dirichlet neighbour
+// [psub' pmid psup' qsub' | qmid qsup' pr qr | S*wsub S*wmid S*wsup snz | S 0 0 0]
+// (p*: first-derivative bands scaled by r S, with -r/2 on the diagonal; q*:
+// raw second-derivative weights, and qr the raw residual, to be scaled by
+// c00 = (1/2 v) S^2 in the kernel) where ' marks band entries zeroed when
+// they multiply a Dirichlet neighbour
 // (row 0 for row 1; row nS-1 for row nS-2 under a barrier) -- the explicit
 // apply forces those neighbours to 0, and in the solve the coupling to a
 // Dirichlet row (whose value is 0) contributes nothing -- and snz = [S != 0].
@@ -336,7 +281,7 @@ __device__ void affine_scan(const double *cM, const double *cE, double *cIn, int
 constexpr int kSpRow = 16;
 
 struct SpotRow {
-  double psub, pmid, psup, qsub, qmid, qsup, pr, qr, wsub, wmid, wsup, snz;
+  double psub, pmid, psup, qsub, qmid, qsup, pr, qr, wsub, wmid, wsup, snz, S;
 };
 
 // 3x3 neighbourhood of a field around (i, j): rows i-1, i, i+1; cols j-1, j, j+1.
@@ -350,16 +295,31 @@ struct VarCol {  // variance coefficients of column j (constant along a spot lin
   double lastGz;  // A1[nv-1, nv-3] on the last column, else 0
 };
 
-// (A0 f)(i,j) from the window (Dirichlet handling is in the bands).
-__device__ __forceinline__ double a0w(const SpotRow &R, double v, const Win &w) {
-  const double sub = R.psub + v * R.qsub, mid = R.pmid + v * R.qmid, sup = R.psup + v * R.qsup;
-  return sub * w.v[0][1] + mid * w.v[1][1] + sup * w.v[2][1];
+// A0 bands of row i at variance v (Dirichlet handling is in the bands).
+struct Bands {
+  double sub, mid, sup, r0;
+};
+
+__device__ __forceinline__ Bands a0_bands(const SpotRow &R, double v) {
+  const double c00 = c00_of(v, R.S);
+  Bands b;
+  b.sub = band_add(R.psub, R.qsub, c00);
+  b.mid = band_add(R.pmid, R.qmid, c00);
+  b.sup = band_add(R.psup, R.qsup, c00);
+  b.r0 = band_add(R.pr, R.qr, c00);  // residual: zero except on the Smax row
+  return b;
+}
+
+// (A0 f)(i,j) from the window.
+__device__ __forceinline__ double a0w(const Bands &B, const Win &w) {
+  return B.sub * w.v[0][1] + B.mid * w.v[1][1] + B.sup * w.v[2][1];
 }
 
 // (A1 f)(i,j); fjm2 = f(i, j-2) (used on the last column only).
 __device__ __forceinline__ double a1w(const HOpt &o, const SpotRow &R, const VarCol &q,
                                       const Win &w, double fjm2) {
-  double acc = (R.snz * q.gmid - 0.5 * o.r) * w.v[1][1];
+  // S = 0 lines keep only the -r/2 of the split
+  double acc = (R.snz != 0.0 ? q.gmid : -0.5 * o.r) * w.v[1][1];
   acc += R.snz * (q.gsub * w.v[1][0] + q.gsup * w.v[1][2] + q.lastGz * fjm2);
   return acc;
 }
@@ -408,6 +368,7 @@ __device__ __forceinline__ SpotRow quad_row_fields(double2 a, double2 b) {
   r.pr = quad_bcast<1>(b.x);   r.qr = quad_bcast<1>(b.y);
   r.wsub = quad_bcast<2>(a.x); r.wmid = quad_bcast<2>(a.y);
   r.wsup = quad_bcast<2>(b.x); r.snz = quad_bcast<2>(b.y);
+  r.S = quad_bcast<3>(a.x);
   return r;
 }
 
@@ -512,7 +473,7 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     const SpotRow Rp = quad_row_fields(tpa, tpb);
     if (s > 0) {
       rdp = rd[clamp_row(s - 1) * unv + ujc];
-      cp = -tdt * (Rp.psup + q.v * Rp.qsup);
+      cp = -tdt * a0_bands(Rp, q.v).sup;
     }
   }
   double M = 1.0, E = 0.0;
@@ -539,12 +500,13 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
         wY.v[r][b] = HAS_Y ? wy[r][b] : 0.0;
       }
     double rhs = 0.0, ex = 0.0;
+    const Bands B = a0_bands(R, q.v);
     if (MODE == SM_IMPL) {
-      rhs = wU.v[1][1] + dt * (R.pr + q.v * R.qr);
+      rhs = wU.v[1][1] + dt * B.r0;
     } else {
-      const double a0u = a0w(R, q.v, wU);
+      const double a0u = a0w(B, wU);
       const double a1u = a1w(o, R, q, wU, uj2);
-      const double r0 = R.pr + q.v * R.qr;  // zero on Dirichlet rows
+      const double r0 = B.r0;
       const double fu = dt * (a0u + r0) + dt * a1u + dt * a01w(R, q, wU);
       rhs = wU.v[1][1] + fu;
       if (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) ex = tdt * a1u;
@@ -552,7 +514,7 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
         rhs -= tdt * a0u;
       } else {
         // Operators are linear: A (Y2 - U) = A Y2 - A U.
-        const double a0y = a0w(R, q.v, wY);
+        const double a0y = a0w(B, wY);
         const double a1y = a1w(o, R, q, wY, yj2);
         const double c01 = a01w(R, q, wY) - a01w(R, q, wU);
         if (MODE == SM_HV) {
@@ -571,9 +533,9 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     if (i == 0 || i == dir_hi) rhs = 0.0;  // the solve forces Dirichlet rows to 0
     // I - tdt*A0 on this row (identity rows on Dirichlet rows and zero
     // off-diagonals at the line ends come from the bands), scaled by the pivot.
-    const double aL = -tdt * (R.psub + q.v * R.qsub);
-    const double bL = 1.0 - tdt * (R.pmid + q.v * R.qmid);
-    const double cL = -tdt * (R.psup + q.v * R.qsup);
+    const double aL = -tdt * B.sub;
+    const double bL = 1.0 - tdt * B.mid;
+    const double cL = -tdt * B.sup;
     const double rd0 = 1.0 / (bL - (aL * rdp) * cp);
     rdp = rd0;
     cp = cL;
@@ -841,11 +803,11 @@ __global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *ro
   col[SP_PSUB * sp_n] = dir ? 0.0 : w1[0] * c0;
   col[SP_PMID * sp_n] = dir ? 0.0 : w1[1] * c0 + (-0.5 * o.r);
   col[SP_PSUP * sp_n] = dir ? 0.0 : w1[2] * c0;
-  col[SP_QSUB * sp_n] = dir ? 0.0 : w2[0] * S2h;
-  col[SP_QMID * sp_n] = dir ? 0.0 : w2[1] * S2h;
-  col[SP_QSUP * sp_n] = dir ? 0.0 : w2[2] * S2h;
+  col[SP_QSUB * sp_n] = dir ? 0.0 : w2[0];
+  col[SP_QMID * sp_n] = dir ? 0.0 : w2[1];
+  col[SP_QSUP * sp_n] = dir ? 0.0 : w2[2];
   col[SP_PR * sp_n] = dir ? 0.0 : R1 * c0;
-  col[SP_QR * sp_n] = dir ? 0.0 : R2 * S2h;
+  col[SP_QR * sp_n] = dir ? 0.0 : R2;
   col[SP_WSUB * sp_n] = ws[0];
   col[SP_WMID * sp_n] = ws[1];
   col[SP_WSUP * sp_n] = ws[2];
@@ -868,7 +830,8 @@ __global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *ro
   row[9] = S * ws[1];
   row[10] = S * ws[2];
   row[11] = S != 0.0 ? 1.0 : 0.0;
-  for (int f = 12; f < 16; ++f) row[f] = 0.0;
+  row[12] = S;
+  for (int f = 13; f < 16; ++f) row[f] = 0.0;
 }
 
 // One thread per variance node: A1 rows for S != 0 lines and cross weights.
@@ -908,7 +871,9 @@ __global__ void var_tables_kernel(const HOpt *opts, const int64_t *row_opt,
   const double c11 = (0.5 * (o.sigma * o.sigma)) * v;
   double *col = vt + g;
   col[VT_GSUB * vt_n] = u1[0] * c1 + u2[0] * c11;
-  col[VT_GMID * vt_n] = u1[1] * c1 + u2[1] * c11;
+  // diagonal of the combined operator for S != 0 lines, in the reference's
+  // order: ((1,) band + split) + (1,1) band
+  col[VT_GMID * vt_n] = (u1[1] * c1 + (-o.r / 2.0)) + u2[1] * c11;
   col[VT_GSUP * vt_n] = u1[2] * c1 + u2[2] * c11;
   col[VT_WSUB * vt_n] = wv[0];
   col[VT_WMID * vt_n] = wv[1];
@@ -934,7 +899,7 @@ __global__ void var_factor_kernel(HOpt *opts, int nopt, const double *__restrict
   const double t = o.tdt;
   auto row = [&](int64_t j, double &a, double &b, double &c) {
     a = j > 0 ? -t * g(VT_GSUB, j) : 0.0;
-    b = 1.0 - t * (g(VT_GMID, j) - 0.5 * o.r);
+    b = 1.0 - t * g(VT_GMID, j);
     c = j < n - 1 ? -t * g(VT_GSUP, j) : 0.0;
   };
   // Fold of the last row: f = -zL / aL[n-2].

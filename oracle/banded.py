@@ -240,7 +240,10 @@ class BandedOperator(object):
         """
         T, back = self._to_lines(V)
         self._force_dirichlet(T)
-        ret = self.D.dot(T.ravel())
+        if getattr(self, "apply_order", None) == "reversed":
+            ret = self._dot_reversed(T.ravel())
+        else:
+            ret = self.D.dot(T.ravel())
         if self.is_folded():
             ret = self.fold_vector(ret, unfold=True)
         if self.R is not None:
@@ -259,8 +262,53 @@ class BandedOperator(object):
             rhs = rhs - self.R
         if self.is_folded():
             rhs = self.fold_vector(rhs)
-        ret = spl.solve_banded(self.solve_banded_offsets, self.data, rhs)
+        if getattr(self, "solver", "lapack") == "thomas" and self.solve_banded_offsets == (1, 1):
+            ret = self._thomas(rhs)
+        else:
+            ret = spl.solve_banded(self.solve_banded_offsets, self.data, rhs)
         return back(ret)
+
+    def _dot_reversed(self, x):
+        """A x summing the diagonals in the reverse of dia_matrix order (a
+        rounding-order probe for tests, like _thomas)."""
+        y = np.zeros_like(x)
+        n = x.shape[0]
+        for k in range(len(self.offsets) - 1, -1, -1):
+            o = int(self.offsets[k])
+            band = self.data[k]
+            # y[i] += A[i, i+o] x[i+o]; DIA stores A[i, i+o] at column j = i + o
+            if o >= 0:
+                y[:n - o] += band[o:] * x[o:]
+            else:
+                y[-o:] += band[:n + o] * x[:n + o]
+        return y
+
+    def _thomas(self, rhs):
+        """Per-block Thomas elimination (no pivoting) of the tridiagonal
+        system, vectorised over the blocks.  Not the reference's algorithm: it
+        exists so that tests can measure how much the rounding order of the
+        implicit solves alone moves a result (the reference solves with LAPACK
+        gbsv, whose operation order differs from any GPU solve)."""
+        nb, L = self.blocks, self.block_len
+        sup = self.data[0].reshape(nb, L)
+        b = self.data[1].reshape(nb, L)
+        sub = self.data[2].reshape(nb, L)
+        d = rhs.reshape(nb, L)
+        cp = np.zeros((nb, L))
+        dp = np.zeros((nb, L))
+        cp[:, 0] = sup[:, 1] / b[:, 0] if L > 1 else 0.0
+        dp[:, 0] = d[:, 0] / b[:, 0]
+        for i in range(1, L):
+            a = sub[:, i - 1]
+            m = b[:, i] - a * cp[:, i - 1]
+            if i < L - 1:
+                cp[:, i] = sup[:, i + 1] / m
+            dp[:, i] = (d[:, i] - a * dp[:, i - 1]) / m
+        x = np.empty((nb, L))
+        x[:, L - 1] = dp[:, L - 1]
+        for i in range(L - 2, -1, -1):
+            x[:, i] = dp[:, i] - cp[:, i] * x[:, i + 1]
+        return x.ravel()
 
     # -- boundaries --------------------------------------------------------------
     def applyboundary(self, boundary, mesh_deltas=None):

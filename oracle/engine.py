@@ -55,7 +55,8 @@ class HestonOracle(object):
     """
 
     def __init__(self, params, spots, vars, r_split_float32=False,
-                 implicit_residual=True, firsts_reversed=False):
+                 implicit_residual=True, firsts_reversed=False, tri_solver="lapack",
+                 apply_order=None):
         self.p = params
         self.spots = np.array(spots, dtype=float)
         self.vars = np.array(vars, dtype=float)
@@ -63,6 +64,11 @@ class HestonOracle(object):
         self.r_split_float32 = r_split_float32
         self.implicit_residual = implicit_residual
         self.firsts_reversed = firsts_reversed
+        # "thomas": solve the implicit systems by per-line Thomas elimination
+        # instead of LAPACK (a rounding-order probe for tests, see banded.py)
+        self.tri_solver = tri_solver
+        # "reversed": every operator apply sums its diagonals in reverse order
+        self.apply_order = apply_order
         self.t = 0.0
         self.simple_operators = {}
         self.operators = {}
@@ -191,14 +197,19 @@ class HestonOracle(object):
         keys = list(self.operators)
         if self.firsts_reversed:
             keys = keys[::-1]
-        return [self.operators[k] * dt for k in keys]
+        return [self._ordered(self.operators[k] * dt) for k in keys]
+
+    def _ordered(self, op):
+        op.apply_order = self.apply_order
+        return op
 
     def _les_lis(self, dt, theta, fold):
-        Les = [self.operators[k] * (theta * dt) for k in self._dims()]
+        Les = [self._ordered(self.operators[k] * (theta * dt)) for k in self._dims()]
         Lis = [(self.operators[k] * (theta * -dt)).add(1, inplace=True)
                for k in self._dims()]
         for L in Les + Lis:
             L.R = None
+            L.solver = self.tri_solver
         if fold:
             for L in Les + Lis:
                 if L.is_foldable():
@@ -208,7 +219,7 @@ class HestonOracle(object):
     def cross_term(self, V):
         """FiniteDifferenceEngine.pyx:595-613 (numpy=False branch)."""
         if (0, 1) in self.operators:
-            return self.operators[(0, 1)].apply(V)
+            return self._ordered(self.operators[(0, 1)]).apply(V)
         return 0.0
 
     # -- schemes ---------------------------------------------------------------------
@@ -217,11 +228,13 @@ class HestonOracle(object):
         solve the variance and the spot systems (order rolled by one)."""
         V = np.array(initial, dtype=float, copy=True)
         Lis = [(self.operators[k] * -dt).add(1, inplace=True) for k in self._dims()]
+        for L in Lis:
+            L.solver = self.tri_solver
         if not self.implicit_residual:
             for L in Lis:
                 L.R = None
         Lis = Lis[1:] + Lis[:1]
-        cross = self.operators[(0, 1)] * dt if (0, 1) in self.operators else None
+        cross = self._ordered(self.operators[(0, 1)] * dt) if (0, 1) in self.operators else None
         for _ in range(int(n)):
             if cross is not None:
                 V = V + cross.apply(V)

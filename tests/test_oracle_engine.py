@@ -90,3 +90,33 @@ def test_float32_rate_quirk_is_small_but_visible():
     a = E.solve_douglas(20, 0.05, U0)[E.idx]
     b = F.solve_douglas(20, 0.05, U0)[E.idx]
     assert a != b and abs(a - b) / abs(a) < 1e-8
+
+
+def test_rounding_order_probes_match_default():
+    """The oracle's rounding-order probes (reversed explicit sum, reversed
+    band products, Thomas elimination instead of LAPACK) compute the same
+    scheme: they agree with the default oracle to rounding on a small grid,
+    and the Thomas solve agrees with a dense solve."""
+    from fd_adi_amd.utils import sinh_space, exponential_space
+    from oracle.engine import HestonOracle, HestonParams
+    sp = sinh_space(100, 500, 7.0, 40)
+    vv = exponential_space(0, 0.04, 0.2, 4.0, 20)
+    hp = HestonParams(spot=100, strike=100, r=0.06, v0=0.04, T=1.0, kappa=1.0, theta=0.04,
+                      sigma=0.3, rho=-0.5)
+    base = HestonOracle(hp, sp, vv)
+    U0 = base.payoff()
+    W = base.solve_craigsneyd(10, 0.05, U0, 0.5)
+    for kw in (dict(firsts_reversed=True), dict(apply_order="reversed"),
+               dict(tri_solver="thomas")):
+        Wp = HestonOracle(hp, sp, vv, **kw).solve_craigsneyd(10, 0.05, U0, 0.5)
+        d = np.max(np.abs(Wp - W)) / np.max(np.abs(W))
+        assert 0 < d < 1e-12, (kw, d)
+    # the Thomas path on one folded variance operator against a dense solve
+    E = HestonOracle(hp, sp, vv, tri_solver="thomas")
+    Les, Lis = E._les_lis(0.05, 0.5, fold=True)
+    Li = Lis[1]
+    rhs = np.random.default_rng(0).normal(size=U0.shape)
+    x_thomas = Li.solve(rhs)
+    Li.solver = "lapack"
+    x_lapack = Li.solve(rhs)
+    npt.assert_allclose(x_thomas, x_lapack, rtol=0, atol=1e-12 * np.max(np.abs(x_lapack)))
