@@ -127,7 +127,7 @@ def batched(n_opt, seed, K, W):
     hb.sync()
     wall = time.perf_counter() - t0
     npts = sum(o.spots.size * o.vars.size for o in opts)
-    return n_opt * K / wall, wall / K, npts
+    return n_opt * K / wall, wall / K, npts, shapes
 
 
 def surrogate_run(n_labels, n_queries, members, steps, seed=17):
@@ -347,7 +347,9 @@ def main():
         extra["plain_150x150_price"] = p0
         loops = single_loops(2, 16)
         extra["single_option_loop_cs_options_per_s"] = loops
-        rate, sec, npts = batched(4096, 2024, 2, 1)
+        rate, sec, npts, shapes = batched(4096, 2024, 2, 1)
+        extra["batched_4096_shapes"] = {"spot_threads_cols": list(shapes[0]),
+                                        "var_lanes": shapes[1][0]}
         extra["batched_4096_cs_options_per_s"] = round(rate, 1)
         extra["batched_4096_cs_sec"] = round(sec, 4)
         extra["batched_4096_cs_GBs_algorithmic"] = round(72 * npts * 200 / sec / 1e9, 1)

@@ -73,18 +73,23 @@ class HestonBatch(object):
         self.scheme = code
         return self
 
-    def set_shape(self, kind, threads):
-        _lib.call("fdadi_hb_set_shape", self._h, int(kind), int(threads), 0)
+    def set_shape(self, kind, threads, lines=0):
+        """Launch shape of a sweep kind (0 = spot: threads per work-group and
+        ``lines`` = variance columns per tile, 4 or 16; 1 = variance: lanes per
+        line); 0 selects the default."""
+        _lib.call("fdadi_hb_set_shape", self._h, int(kind), int(threads), int(lines))
 
-    # launch-shape candidates per sweep kind (spot: threads per work-group;
-    # variance: lanes per line); set_shape rejects those too small for the grids
-    SHAPES = {0: (64, 128, 256, 512, 1024), 1: (1, 2, 4, 8, 16, 32, 64)}
+    # launch-shape candidates per sweep kind (spot: (threads per work-group,
+    # columns per tile); variance: (lanes per line, 0)); set_shape rejects
+    # those too small for the grids
+    SHAPES = {0: ((64, 4), (128, 4), (256, 4), (512, 4), (1024, 4), (256, 16), (512, 16)),
+              1: tuple((t, 0) for t in (1, 2, 4, 8, 16, 32, 64))}
 
     def autotune(self, steps=2):
         """Pick the fastest launch shape of each sweep kind for the configured
         scheme: every valid candidate is timed over ``steps`` steps with HIP
         events (the stage slots of that kind summed); the state is saved and
-        restored.  Returns {kind: threads} as set."""
+        restored.  Returns {kind: (threads, lines)} as set."""
         if self.scheme is None:
             raise RuntimeError("autotune: configure the engine first")
         # stage slots of each kind in a step (fdadi_hb_step's launch order)
@@ -95,14 +100,14 @@ class HestonBatch(object):
             timings = []
             for c in self.SHAPES[kind]:
                 try:
-                    self.set_shape(kind, c)
+                    self.set_shape(kind, *c)
                 except (ValueError, NotImplementedError):
                     continue
                 self.step_timed(1)
                 st = self.step_timed(steps)
                 timings.append((sum(t for t, k in zip(st, order) if k == kind), c))
             best[kind] = min(timings)[1]
-            self.set_shape(kind, best[kind])
+            self.set_shape(kind, *best[kind])
         self.state_view().copy_from(saved)
         self.sync()
         return best

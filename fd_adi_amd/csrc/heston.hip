@@ -372,6 +372,49 @@ __device__ __forceinline__ SpotRow quad_row_fields(double2 a, double2 b) {
   return r;
 }
 
+// Broadcast lane Q of each row of 16 lanes (DPP row_newbcast:Q).
+template <int Q>
+__device__ __forceinline__ double row16_bcast(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x150 + Q, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x150 + Q, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// The same row shared by the 16 lanes of a DPP row (16-column tiles): lane jj
+// loads 16-byte piece jj % 8 of the 128-byte row (lanes 8-15 repeat lanes
+// 0-7), and field f is broadcast from lane f / 2.
+__device__ __forceinline__ SpotRow row16_fields(double2 a) {
+  SpotRow r;
+  r.psub = row16_bcast<0>(a.x); r.pmid = row16_bcast<0>(a.y);
+  r.psup = row16_bcast<1>(a.x); r.qsub = row16_bcast<1>(a.y);
+  r.qmid = row16_bcast<2>(a.x); r.qsup = row16_bcast<2>(a.y);
+  r.pr = row16_bcast<3>(a.x);   r.qr = row16_bcast<3>(a.y);
+  r.wsub = row16_bcast<4>(a.x); r.wmid = row16_bcast<4>(a.y);
+  r.wsup = row16_bcast<5>(a.x); r.snz = row16_bcast<5>(a.y);
+  r.S = row16_bcast<6>(a.x);
+  return r;
+}
+
+// Table-row share of lane jj for tiles of NL columns (see above).
+template <int NL>
+__device__ __forceinline__ void share_row_load(const double *__restrict__ spr, int jj,
+                                               double2 &a, double2 &b) {
+  if constexpr (NL == 4) {
+    quad_row_load(spr, jj, a, b);
+  } else {
+    a = reinterpret_cast<const double2 *>(spr)[jj & 7];
+    b = a;
+  }
+}
+
+template <int NL>
+__device__ __forceinline__ SpotRow share_row_fields(double2 a, double2 b) {
+  if constexpr (NL == 4)
+    return quad_row_fields(a, b);
+  else
+    return row16_fields(a);
+}
+
 // Row rr (already clamped to the grid) of a field at columns jm, j, jp, and
 // at column jm2 = j - 2 on the last column only (the second-order backward
 // row of d/dv reaches two columns back; elsewhere out[3] = 0).  Offsets are
@@ -386,10 +429,14 @@ __device__ __forceinline__ void load_row32(const double *__restrict__ f, unsigne
   out[3] = last ? f[b + jm2] : 0.0;
 }
 
-// A spot sweep: one work-group per (option, NL = 4 adjacent variance
-// columns); thread (jj, ch) = (tid % 4, tid / 4) owns rows [ch*L, ch*L+L) of
-// column j0+jj through every phase.  The four lanes of a quad work on the
-// same rows, so they share the spot-table rows.  The row walk keeps its loads
+// A spot sweep: one work-group per (option, NL adjacent variance columns,
+// NL = 4 or 16); thread (jj, ch) = (tid % NL, tid / NL) owns rows
+// [ch*L, ch*L+L) of column j0+jj through every phase.  The NL lanes of a quad
+// (NL = 4) or of a DPP row (NL = 16) work on the same rows, so they share the
+// spot-table rows.  Four columns per tile keep a large grid spread over every
+// CU (2048 x 1024: 256 tiles); sixteen make each load of a wave touch 4 rows
+// of 128 bytes instead of 16 rows of 32 bytes, which pays where the batch has
+// tiles to spare.  The row walk keeps its loads
 // two rows ahead of the arithmetic (table rows, pivots, and the state rows
 // entering the 3x3 windows); a scheduling barrier per row stops the compiler
 // from hoisting more than that, which would exhaust the registers.  Loads use
@@ -406,7 +453,7 @@ template <int MODE, int THREADS, int NL, int MAXL>
 __global__ void __launch_bounds__(THREADS)
 spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ U,
                   const double *__restrict__ Y, double *__restrict__ out) {
-  static_assert(NL == 4, "spot tiles are one quad of columns wide");
+  static_assert(NL == 4 || NL == 16, "spot tiles are a quad or a DPP row of columns wide");
   extern __shared__ double lds[];
   constexpr int TPL = THREADS / NL;
   constexpr bool HAS_Y = MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS;
@@ -461,16 +508,16 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
       load_row32(y2, clamp_row(s - 1 + r), unv, jm, ujc, jp, jm2, last, wy[r]);
   }
   double2 ta0, tb0, ta1, tb1;  // table rows i and i+1
-  quad_row_load(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
-  quad_row_load(spr + clamp_row(s + 1) * kSpRow, jj, ta1, tb1);
+  share_row_load<NL>(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
+  share_row_load<NL>(spr + clamp_row(s + 1) * kSpRow, jj, ta1, tb1);
   // Pivots: rd_i = 1 / (b_i - (a_i rd_{i-1}) c_{i-1}) down the chunk, from the
   // stored pivot and super-diagonal of the row before the chunk (the same
   // recurrence the factorisation ran).
   double rdp = 0.0, cp = 0.0;
   {
     double2 tpa, tpb;
-    quad_row_load(spr + clamp_row(s - 1) * kSpRow, jj, tpa, tpb);
-    const SpotRow Rp = quad_row_fields(tpa, tpb);
+    share_row_load<NL>(spr + clamp_row(s - 1) * kSpRow, jj, tpa, tpb);
+    const SpotRow Rp = share_row_fields<NL>(tpa, tpb);
     if (s > 0) {
       rdp = rd[clamp_row(s - 1) * unv + ujc];
       cp = -tdt * a0_bands(Rp, q.v).sup;
@@ -483,14 +530,14 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     // loads for two rows ahead
     double2 ta2, tb2;
     const unsigned i2 = clamp_row(i + 2), i3 = clamp_row(i + 3);
-    quad_row_load(spr + i2 * kSpRow, jj, ta2, tb2);
+    share_row_load<NL>(spr + i2 * kSpRow, jj, ta2, tb2);
     double nu[4], ny[4];
     load_row32(u, i3, unv, jm, ujc, jp, jm2, last, nu);
     if (HAS_Y) load_row32(y2, i3, unv, jm, ujc, jp, jm2, last, ny);
     const double uj2 = wu[1][3];
     const double yj2 = HAS_Y ? wy[1][3] : 0.0;
 
-    const SpotRow R = quad_row_fields(ta0, tb0);
+    const SpotRow R = share_row_fields<NL>(ta0, tb0);
     Win wU, wY;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -975,6 +1022,7 @@ struct fdadi_hb {
   int var_tpl = 1, var_rows = 256;  // variance sweep: lanes per line, rows per 256-thread tile
   std::vector<int> order;           // options, largest first
   int spot_maxl = 8;  // rows per thread in the spot sweep (8, or 16 for long lines)
+  int spot_nl = 4;    // columns per spot-sweep tile (4 or 16)
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
   hipGraph_t graph = nullptr;
@@ -987,7 +1035,6 @@ namespace {
 
 constexpr int kNL = 8;     // variance sweep: rows per tile
 constexpr int kMAXL = 16;  // variance sweep: positions per thread
-constexpr int kNLs = 4;    // spot sweep: columns per tile
 constexpr int kMAXLs = 8;  // spot sweep: rows per thread
 
 Ctx ctx_of(const fdadi_hb *hb) {
@@ -1008,33 +1055,55 @@ Ctx ctx_of(const fdadi_hb *hb) {
     if (_st) return _st;     \
   } while (0)
 
-// Spot-sweep shape: `threads` (0 = default) and rows per thread for lines of
-// length n.  Lines up to 1024 use 8 rows per thread; longer lines 16 rows per
-// thread at 512 threads (1024 x 8 would exceed the 128-VGPR budget).
-int spot_shape(int64_t n, int threads, int *t_out, int *maxl_out) {
+// Spot-sweep shape for lines of length n: `threads` per work-group (0 =
+// default), `nl` columns per tile (4 or 16), and the rows per thread (8, or
+// 16 for long lines).  The register budget allows 16 rows per thread only at
+// <= 512 threads (two waves per SIMD).
+int spot_shape(int64_t n, int threads, int nl, int *t_out, int *maxl_out) {
+  if (nl != 4 && nl != 16) {
+    set_error("fdadi_hb_set_shape: spot tiles are 4 or 16 columns wide");
+    return FDADI_ERR_ARG;
+  }
   if (threads == 0) {
-    if (n <= 1024) {
-      const int64_t need = (n + kMAXLs - 1) / kMAXLs * kNLs;
+    if (nl == 4 && n <= 1024) {
+      const int64_t need = (n + kMAXLs - 1) / kMAXLs * 4;
       int t = 64;
       while (t < need) t *= 2;
       *t_out = t;
       *maxl_out = kMAXLs;
-    } else {
-      *t_out = 512;
-      *maxl_out = 2 * kMAXLs;
+      return FDADI_OK;
     }
-    return FDADI_OK;
+    threads = 512;
   }
-  if ((int64_t)(threads / kNLs) * kMAXLs >= n && threads <= 1024) {
+  const int tpl = threads / nl;
+  const bool t_ok = threads >= 64 && threads <= 1024 && (threads & (threads - 1)) == 0 &&
+                    tpl >= 1;
+  if (t_ok && (int64_t)tpl * kMAXLs >= n && !(nl == 16 && threads > 512)) {
     *t_out = threads;
     *maxl_out = kMAXLs;
-  } else if ((int64_t)(threads / kNLs) * 2 * kMAXLs >= n && (threads == 256 || threads == 512)) {
+  } else if (t_ok && (int64_t)tpl * 2 * kMAXLs >= n && (threads == 256 || threads == 512)) {
     *t_out = threads;
     *maxl_out = 2 * kMAXLs;
   } else {
     set_error("fdadi_hb_set_shape: too few threads for the longest spot line");
     return FDADI_ERR_ARG;
   }
+  return FDADI_OK;
+}
+
+// (Re)build the spot-sweep tile list: spot_nl columns per tile.
+int build_spot_tiles(fdadi_hb *hb) {
+  std::vector<Tile> st;
+  for (int k : hb->order) {
+    const HOpt &o = hb->h_opts[k];
+    for (int64_t j0 = 0; j0 < o.nv; j0 += hb->spot_nl) st.push_back(Tile{k, (int32_t)j0});
+  }
+  if (hb->d_stiles) FDADI_TRY_HIP(hipFree(hb->d_stiles));
+  hb->d_stiles = nullptr;
+  FDADI_TRY_HIP(hipMalloc((void **)&hb->d_stiles, sizeof(Tile) * st.size()));
+  FDADI_TRY_HIP(hipMemcpy(hb->d_stiles, st.data(), sizeof(Tile) * st.size(),
+                          hipMemcpyHostToDevice));
+  hb->nstiles = (int)st.size();
   return FDADI_OK;
 }
 
@@ -1078,14 +1147,14 @@ int threads_for(int64_t n, int nl = kNL, int maxl = kMAXL) {
   return t;
 }
 
-template <int MODE, int T, int ML = kMAXLs>
+template <int MODE, int T, int NL, int ML>
 int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                   hipStream_t s) {
-  constexpr int TPL = T / kNLs;
-  const size_t lds = sizeof(double) * (2 * ML * T + 3 * TPL * kNLs);
-  FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, kNLs, ML>, lds));
-  hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, kNLs, ML>), dim3(hb->nstiles), dim3(T), lds,
-                     s, ctx_of(hb), hb->d_stiles, U, Y, out);
+  constexpr int TPL = T / NL;
+  const size_t lds = sizeof(double) * (2 * ML * T + 3 * TPL * NL);
+  FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, NL, ML>, lds));
+  hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, NL, ML>), dim3(hb->nstiles), dim3(T), lds, s,
+                     ctx_of(hb), hb->d_stiles, U, Y, out);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
@@ -1093,20 +1162,31 @@ int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *
 template <int MODE>
 int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                 hipStream_t s) {
-  if (hb->spot_maxl == 2 * kMAXLs) {
+  const bool long_rows = hb->spot_maxl == 2 * kMAXLs;
+  if (hb->spot_nl == 16) {
+    switch (hb->spot_threads * 2 + (long_rows ? 1 : 0)) {
+      case 512: return launch_spot_t<MODE, 256, 16, kMAXLs>(hb, U, Y, out, s);
+      case 513: return launch_spot_t<MODE, 256, 16, 2 * kMAXLs>(hb, U, Y, out, s);
+      case 1024: return launch_spot_t<MODE, 512, 16, kMAXLs>(hb, U, Y, out, s);
+      case 1025: return launch_spot_t<MODE, 512, 16, 2 * kMAXLs>(hb, U, Y, out, s);
+    }
+    set_error("spot sweep: unsupported shape for 16-column tiles");
+    return FDADI_ERR_UNSUPPORTED;
+  }
+  if (long_rows) {
     switch (hb->spot_threads) {
-      case 256: return launch_spot_t<MODE, 256, 2 * kMAXLs>(hb, U, Y, out, s);
-      case 512: return launch_spot_t<MODE, 512, 2 * kMAXLs>(hb, U, Y, out, s);
+      case 256: return launch_spot_t<MODE, 256, 4, 2 * kMAXLs>(hb, U, Y, out, s);
+      case 512: return launch_spot_t<MODE, 512, 4, 2 * kMAXLs>(hb, U, Y, out, s);
     }
     set_error("spot sweep: unsupported thread count for 16 rows per thread");
     return FDADI_ERR_UNSUPPORTED;
   }
   switch (hb->spot_threads) {
-    case 64: return launch_spot_t<MODE, 64>(hb, U, Y, out, s);
-    case 128: return launch_spot_t<MODE, 128>(hb, U, Y, out, s);
-    case 256: return launch_spot_t<MODE, 256>(hb, U, Y, out, s);
-    case 512: return launch_spot_t<MODE, 512>(hb, U, Y, out, s);
-    case 1024: return launch_spot_t<MODE, 1024>(hb, U, Y, out, s);
+    case 64: return launch_spot_t<MODE, 64, 4, kMAXLs>(hb, U, Y, out, s);
+    case 128: return launch_spot_t<MODE, 128, 4, kMAXLs>(hb, U, Y, out, s);
+    case 256: return launch_spot_t<MODE, 256, 4, kMAXLs>(hb, U, Y, out, s);
+    case 512: return launch_spot_t<MODE, 512, 4, kMAXLs>(hb, U, Y, out, s);
+    case 1024: return launch_spot_t<MODE, 1024, 4, kMAXLs>(hb, U, Y, out, s);
   }
   set_error("spot sweep: unsupported thread count");
   return FDADI_ERR_UNSUPPORTED;
@@ -1270,14 +1350,14 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   std::vector<Tile> st, vtl, ft;
   for (int k : order) {
     const HOpt &o = hb->h_opts[k];
-    for (int64_t j0 = 0; j0 < o.nv; j0 += kNLs) st.push_back(Tile{k, (int32_t)j0});
+    for (int64_t j0 = 0; j0 < o.nv; j0 += 4) st.push_back(Tile{k, (int32_t)j0});
 
     for (int64_t j0 = 0; j0 < o.nv; j0 += 64) ft.push_back(Tile{k, (int32_t)j0});
   }
   hb->nstiles = (int)st.size();
 
   hb->nftiles = (int)ft.size();
-  spot_shape(hb->nSmax, 0, &hb->spot_threads, &hb->spot_maxl);
+  spot_shape(hb->nSmax, 0, 4, &hb->spot_threads, &hb->spot_maxl);
   hb->order = order;
   hb->var_tpl = var_tpl_for(hb->nvmax);
   hb->var_rows = 256 / hb->var_tpl;
@@ -1464,14 +1544,22 @@ int fdadi_hb_replay(fdadi_hb *hb, void *stream) {
 
 int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
   FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_set_shape: bad kind");
-  FDADI_REQUIRE(lines == 0, FDADI_ERR_UNSUPPORTED,
-                "fdadi_hb_set_shape: lines per tile follow from the shape (pass 0)");
+  FDADI_REQUIRE(kind == 0 || lines == 0, FDADI_ERR_UNSUPPORTED,
+                "fdadi_hb_set_shape: variance tiles follow from the lanes per line (pass 0)");
   if (kind == 0) {
     FDADI_REQUIRE(threads == 0 || threads == 64 || threads == 128 || threads == 256 ||
                       threads == 512 || threads == 1024,
                   FDADI_ERR_ARG,
                   "fdadi_hb_set_shape: threads must be a power of two in [64, 1024]");
-    FDADI_RET(spot_shape(hb->nSmax, threads, &hb->spot_threads, &hb->spot_maxl));
+    const int nl = lines == 0 ? 4 : lines;
+    int t = 0, ml = 0;
+    FDADI_RET(spot_shape(hb->nSmax, threads, nl, &t, &ml));
+    hb->spot_threads = t;
+    hb->spot_maxl = ml;
+    if (nl != hb->spot_nl) {
+      hb->spot_nl = nl;
+      FDADI_RET(build_spot_tiles(hb));
+    }
   } else {
     // kind 1: `threads` = lanes per variance line (power of two <= 64)
     const int t = threads == 0 ? var_tpl_for(hb->nvmax) : threads;
@@ -1497,7 +1585,7 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
 int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines) {
   FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_get_shape: bad kind");
   if (threads) *threads = kind == 0 ? hb->spot_threads : hb->var_tpl;
-  if (lines) *lines = kind == 0 ? kNLs : hb->var_rows;
+  if (lines) *lines = kind == 0 ? hb->spot_nl : hb->var_rows;
   return FDADI_OK;
 }
 

@@ -83,11 +83,12 @@ def test_graph_replay_equals_eager(fd):
 def test_launch_shapes_agree(fd):
     P, S, Vs, dts, O = draw(3, 3, nS_range=(40, 100), nv_range=(20, 60))
     outs = []
-    for ts, tv in ((0, 0), (256, 16), (512, 64)):
+    for ts, nl, tv in ((0, 0, 0), (256, 4, 16), (512, 4, 64), (256, 16, 8), (512, 16, 32)):
         hb = fd.HestonBatch(P, S, Vs, dts)
         if ts:
-            hb.set_shape(0, ts)
+            hb.set_shape(0, ts, nl)
             hb.set_shape(1, tv)
+            assert hb.get_shape(0) == (ts, nl)
         hb.set_state([o.payoff() for o in O])
         hb.configure("hv", 0.5)
         hb.step(10)
@@ -162,7 +163,7 @@ def test_autotune_restores_state_and_keeps_results(fd):
     shapes = tuned.autotune(steps=1)
     assert set(shapes) == {0, 1}
     npt.assert_array_equal(tuned.get_packed(), before)
-    assert tuned.get_shape(0)[0] == shapes[0] and tuned.get_shape(1)[0] == shapes[1]
+    assert tuned.get_shape(0) == shapes[0] and tuned.get_shape(1)[0] == shapes[1][0]
     tuned.step(6)
     scale = np.max(np.abs(ref.get_packed()))
     assert np.max(np.abs(tuned.get_packed() - ref.get_packed())) <= 1e-12 * scale
