@@ -35,6 +35,7 @@ constexpr int kH = 512;      // hidden width
 constexpr int kLayers = 4;   // hidden layers (the 512 -> 1 head is separate)
 constexpr int kTile = 128;
 constexpr int kKStep = 32;
+constexpr int kMaxSplits = 16;  // split-K factor cap for the weight-gradient GEMMs
 constexpr int kLdsPitch = kKStep + 8;  // bf16 elements per LDS row (16 B pad)
 
 enum Epi { EPI_FWD = 0, EPI_INF = 1, EPI_BWD = 2, EPI_F32 = 3 };
@@ -68,6 +69,8 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int wm = wave / 2, wn = wave % 2;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  // split-K: work-group z covers K columns [z*Ks, (z+1)*Ks) (gridDim.z = 1: all)
+  const int Ks = K / (int)gridDim.z, kbeg = (int)blockIdx.z * Ks, kend = kbeg + Ks;
   f32x4 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -75,7 +78,7 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
     for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
   // Global -> LDS: each thread moves 2 x 16 B of A and of B per K step.
   const int lrow = tid / 4, lk = (tid % 4) * 8;
-  for (int k0 = 0; k0 < K; k0 += kKStep) {
+  for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       const int r = lrow + 64 * p;
@@ -113,8 +116,10 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
       const int mb = m0 + wm * 64 + a * 16 + 4 * (lane / 16);
       float v[4] = {acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
       if (EPI == EPI_F32) {
+        // split z writes its partial product to slice z of C
+        float *Cz = ep.C + (size_t)blockIdx.z * M * N;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) ep.C[(size_t)(mb + t) * N + n] = v[t];
+        for (int t = 0; t < 4; ++t) Cz[(size_t)(mb + t) * N + n] = v[t];
         continue;
       }
       bf16 o[4];
@@ -232,6 +237,16 @@ __global__ void bias_grad_kernel(const bf16 *__restrict__ dZT, int Mpad, float *
   if (threadIdx.x == 0) gb[n] = red[0];
 }
 
+// out[i] = sum over the splits of part[z*n + i], in split order (deterministic).
+__global__ void splitk_sum_kernel(const float *__restrict__ part, int splits, int64_t n,
+                                  float *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int z = 0; z < splits; ++z) acc += part[(size_t)z * n + i];
+  out[i] = acc;
+}
+
 // Adam on the flat fp32 master parameters.
 __global__ void adam_kernel(float *p, const float *__restrict__ g, float *m, float *v, int64_t n,
                             float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -285,6 +300,7 @@ struct fdadi_mlp {
   float *Z[kLayers] = {};
   bf16 *dZ = nullptr, *dZT = nullptr, *dZ2 = nullptr, *dZT2 = nullptr;
   float *y = nullptr, *dy = nullptr, *loss = nullptr;
+  float *wpart = nullptr;  // split-K partial weight gradients [kMaxSplits x kH x kH]
   double *lohi = nullptr;  // [lo(8) | hi(8)]
   double tmean = 0.0, tstd = 1.0;
   int64_t step = 0;
@@ -296,10 +312,12 @@ namespace {
 int in_dim(int l) { return l == 0 ? kInPad : kH; }
 
 int gemm(int epi, const bf16 *A, const bf16 *B, int M, int N, int K, const EpiArgs &ep,
-         hipStream_t s) {
+         hipStream_t s, int splits = 1) {
   FDADI_REQUIRE(M % kTile == 0 && N % kTile == 0 && K % kKStep == 0, FDADI_ERR_ARG,
                 "mlp gemm: dimensions must be multiples of 128 x 128 x 32");
-  dim3 grid(N / kTile, M / kTile);
+  FDADI_REQUIRE(splits >= 1 && (splits == 1 || epi == EPI_F32) && K % (kKStep * splits) == 0,
+                FDADI_ERR_ARG, "mlp gemm: split-K needs the fp32 epilogue and K % (32 s) == 0");
+  dim3 grid(N / kTile, M / kTile, splits);
   switch (epi) {
     case EPI_FWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
     case EPI_INF: hipLaunchKernelGGL(gemm_nt_kernel<EPI_INF>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
@@ -400,6 +418,7 @@ int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
   al((void **)&m->y, sizeof(float) * R);
   al((void **)&m->dy, sizeof(float) * R);
   al((void **)&m->loss, sizeof(float));
+  al((void **)&m->wpart, sizeof(float) * kMaxSplits * kH * kH);
   al((void **)&m->lohi, sizeof(double) * 2 * kIn);
   if (e == hipSuccess) e = hipMemcpy(m->p, init_params, sizeof(float) * o, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->lohi, lo, sizeof(double) * kIn, hipMemcpyHostToDevice);
@@ -416,7 +435,7 @@ int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
     set_error(std::string("fdadi_mlp_create: ") + hipGetErrorString(e));
     // fall through to free what was allocated
     void *ptrs[] = {m->p, m->g, m->m, m->v, m->w4b, m->X, m->XT, m->dZ, m->dZT, m->dZ2,
-                    m->dZT2, m->y, m->dy, m->loss, m->lohi};
+                    m->dZT2, m->y, m->dy, m->loss, m->wpart, m->lohi};
     for (void *q : ptrs) (void)hipFree(q);
     for (int l = 0; l < kLayers; ++l) {
       (void)hipFree(m->Wb[l]); (void)hipFree(m->WbT[l]); (void)hipFree(m->Act[l]);
@@ -432,7 +451,7 @@ int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
 int fdadi_mlp_destroy(fdadi_mlp *m) {
   if (!m) return FDADI_OK;
   void *ptrs[] = {m->p, m->g, m->m, m->v, m->w4b, m->X, m->XT, m->dZ, m->dZT, m->dZ2,
-                  m->dZT2, m->y, m->dy, m->loss, m->lohi};
+                  m->dZT2, m->y, m->dy, m->loss, m->wpart, m->lohi};
   for (void *q : ptrs) (void)hipFree(q);
   for (int l = 0; l < kLayers; ++l) {
     (void)hipFree(m->Wb[l]); (void)hipFree(m->WbT[l]); (void)hipFree(m->Act[l]);
@@ -485,9 +504,21 @@ static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const i
   for (int l = kLayers - 1; l >= 0; --l) {
     // dW_l = dZ_l^T A_{l-1};  db_l = column sums of dZ_l
     const bf16 *aT = l == 0 ? m->XT : m->ActT[l - 1];
+    // (kH x in_dim) output tiles are few; the long K = Mpad is split so that
+    // the grid fills the GPU, and the partial products are summed in order
+    int splits = 1;
+    while (splits < kMaxSplits && Mpad % (kKStep * splits * 2) == 0 &&
+           Mpad / (splits * 2) >= 512)
+      splits *= 2;
     EpiArgs ew{};
-    ew.C = m->g + m->off_W[l];
-    FDADI_RET(gemm(EPI_F32, dzt, aT, kH, in_dim(l), Mpad, ew, s));
+    ew.C = splits > 1 ? m->wpart : m->g + m->off_W[l];
+    FDADI_RET(gemm(EPI_F32, dzt, aT, kH, in_dim(l), Mpad, ew, s, splits));
+    if (splits > 1) {
+      const int64_t nw = (int64_t)kH * in_dim(l);
+      hipLaunchKernelGGL(splitk_sum_kernel, dim3(grid_for(nw, 256)), dim3(256), 0, s, m->wpart,
+                         splits, nw, m->g + m->off_W[l]);
+      FDADI_CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL(bias_grad_kernel, dim3(kH), dim3(256), 0, s, dzt, Mpad,
                        m->g + m->off_b[l]);
     FDADI_CHECK_LAUNCH();

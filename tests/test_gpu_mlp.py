@@ -35,13 +35,14 @@ def test_gemm_nt_matches_numpy(fd):
     npt.assert_allclose(got, want, rtol=1e-5, atol=1e-4)
 
 
-def test_gradients_match_reference(fd):
+@pytest.mark.parametrize("B", [256, 2048])
+def test_gradients_match_reference(fd, B):
+    """B = 2048 exercises the split-K weight-gradient GEMMs (4 splits)."""
     from fd_adi_amd.surrogate import MLP, init_params
     from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
     rng = np.random.default_rng(1)
     lo, hi = np.zeros(8), np.ones(8)
     p = init_params(3)
-    B = 256
     P = rng.uniform(0, 1, (B, 8))
     t = np.sin(P.sum(axis=1))
     m = MLP(lo, hi, 0.0, 1.0, params=p, max_rows=B)
