@@ -64,8 +64,11 @@ template <int EPI>
 __global__ void __launch_bounds__(256)
 gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, int N, int K,
                EpiArgs ep) {
-  __shared__ __attribute__((aligned(16))) bf16 As[kTile * kLdsPitch];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[kTile * kLdsPitch];
+  // Two LDS buffers: the MFMAs of K step k read buffer k % 2 while the global
+  // loads of step k + 1 are in flight; those then go to the other buffer, so
+  // one barrier per step suffices.
+  __shared__ __attribute__((aligned(16))) bf16 As[2][kTile * kLdsPitch];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][kTile * kLdsPitch];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int wm = wave / 2, wn = wave % 2;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
@@ -78,34 +81,50 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
     for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
   // Global -> LDS: each thread moves 2 x 16 B of A and of B per K step.
   const int lrow = tid / 4, lk = (tid % 4) * 8;
-  for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
+  bf16x8 ra[2], rb[2];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       const int r = lrow + 64 * p;
-      const bf16x8 va = *reinterpret_cast<const bf16x8 *>(A + (size_t)(m0 + r) * K + k0 + lk);
-      const bf16x8 vb = *reinterpret_cast<const bf16x8 *>(B + (size_t)(n0 + r) * K + k0 + lk);
-      *reinterpret_cast<bf16x8 *>(As + r * kLdsPitch + lk) = va;
-      *reinterpret_cast<bf16x8 *>(Bs + r * kLdsPitch + lk) = vb;
+      ra[p] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(m0 + r) * K + k0 + lk);
+      rb[p] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(n0 + r) * K + k0 + lk);
     }
-    __syncthreads();
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int r = lrow + 64 * p;
+      *reinterpret_cast<bf16x8 *>(As[buf] + r * kLdsPitch + lk) = ra[p];
+      *reinterpret_cast<bf16x8 *>(Bs[buf] + r * kLdsPitch + lk) = rb[p];
+    }
+  };
+  fetch(kbeg);
+  stage(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
+    const bool more = k0 + kKStep < kend;
+    if (more) fetch(k0 + kKStep);
     bf16x8 af[4], bfr[4];
     const int kseg = (lane / 16) * 8;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const int row = wm * 64 + a * 16 + lane % 16;
-      af[a] = *reinterpret_cast<const bf16x8 *>(As + row * kLdsPitch + kseg);
+      af[a] = *reinterpret_cast<const bf16x8 *>(As[buf] + row * kLdsPitch + kseg);
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int col = wn * 64 + b * 16 + lane % 16;
-      bfr[b] = *reinterpret_cast<const bf16x8 *>(Bs + col * kLdsPitch + kseg);
+      bfr[b] = *reinterpret_cast<const bf16x8 *>(Bs[buf] + col * kLdsPitch + kseg);
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b)
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    if (more) stage(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
   // Epilogue.
 #pragma unroll
