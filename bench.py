@@ -266,6 +266,20 @@ def probe():
     log("probe: 20 HV steps (80 sweep launches) + 5 scale launches on 2^25 doubles done")
 
 
+def probe_mlp():
+    """Fixed MLP dispatch sequence for kernel traces (see --probe-mlp)."""
+    import numpy as np
+    import fd_adi_amd
+    from fd_adi_amd import surrogate as S
+    rng = np.random.default_rng(0)
+    P = S.draw_params(8192, 1)
+    y = rng.normal(10.0, 5.0, 8192)
+    m = S.MLP(S.BOX_LO, S.BOX_HI, 10.0, 5.0, seed=0, max_rows=8192)
+    m.fit(P, y, 50, batch=8192, lr=1e-3)
+    fd_adi_amd._lib.sync()
+    log("probe-mlp: 50 training steps at batch 8192 done")
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 summary."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_config1.txt")
@@ -295,6 +309,9 @@ def main():
                          "2048x1024 option, then 5 launches of the scale kernel on 2^25 "
                          "doubles (reads and writes 256 MiB each: the copy reference); "
                          "prints no JSON line")
+    ap.add_argument("--probe-mlp", action="store_true",
+                    help="profiling probe: 50 Adam steps of one surrogate network at batch "
+                         "8192 on random features and targets; prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
@@ -306,6 +323,9 @@ def main():
 
     if args.probe:
         probe()
+        return
+    if args.probe_mlp:
+        probe_mlp()
         return
     opt = config1()
     N = opt.spots.size * opt.vars.size

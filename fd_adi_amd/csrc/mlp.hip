@@ -196,39 +196,60 @@ __global__ void head_forward_kernel(const bf16 *__restrict__ Aact, const bf16 *_
 
 // Loss gradient and the head's backward: dy = 2 (y - t) / B (rows >= valid
 // are padding with dy = 0); dZ4 = dy w silu'(Z4) (bf16, and transposed).
-__global__ void head_backward_kernel(const float *__restrict__ y, const double *__restrict__ T,
-                                     const int *__restrict__ idx, int valid, int Mpad,
-                                     const bf16 *__restrict__ w, const float *__restrict__ Z,
-                                     float *dy, bf16 *dZ, bf16 *dZT, float *loss_acc,
-                                     double tmean, double tstd) {
-  const int r = blockIdx.x, n0 = threadIdx.x;
-  float g = 0.f;
-  if (r < valid) {
-    const float t = (float)((T[idx ? idx[r] : r] - tmean) / tstd);
-    const float e = y[r] - t;
-    g = 2.f * e / (float)valid;
-    if (n0 == 0) {
-      dy[r] = g;
-      atomicAdd(loss_acc, e * e / (float)valid);
+// Head backward on a 64 x 64 tile of (rows, features): dy = 2 (y - t) / n
+// per row, dZ = dy w4 silu'(Z) (row-major) and its transpose dZT, staged in
+// LDS so that both stores are coalesced; one loss partial per tile row block.
+__global__ void __launch_bounds__(256)
+head_backward_kernel(const float *__restrict__ y, const double *__restrict__ T,
+                     const int *__restrict__ idx, int valid, int Mpad,
+                     const bf16 *__restrict__ w, const float *__restrict__ Z, float *dy,
+                     bf16 *dZ, bf16 *dZT, float *loss_acc, double tmean, double tstd) {
+  __shared__ float gs[64];
+  __shared__ float es[64];
+  __shared__ bf16 tile[64][66];
+  const int r0 = blockIdx.y * 64, n0 = blockIdx.x * 64, tid = threadIdx.x;
+  if (tid < 64) {
+    const int r = r0 + tid;
+    float g = 0.f, e2 = 0.f;
+    if (r < valid) {
+      const float t = (float)((T[idx ? idx[r] : r] - tmean) / tstd);
+      const float e = y[r] - t;
+      g = 2.f * e / (float)valid;
+      e2 = e * e / (float)valid;
     }
-  } else if (n0 == 0) {
-    dy[r] = 0.f;
+    gs[tid] = g;
+    es[tid] = e2;
+    if (blockIdx.x == 0) dy[r] = g;
   }
-  for (int n = n0; n < kH; n += blockDim.x) {
-    const size_t i = (size_t)r * kH + n;
-    const bf16 v = static_cast<bf16>(g * (float)w[n] * dsilu(Z[i]));
+  __syncthreads();
+  if (blockIdx.x == 0 && tid == 0) {
+    float l = 0.f;
+    for (int k = 0; k < 64; ++k) l += es[k];
+    atomicAdd(loss_acc, l);
+  }
+  // 64 x 64 elements, 16 per thread: consecutive threads along the features
+  for (int q = tid; q < 64 * 64; q += 256) {
+    const int rr = q / 64, nn = q % 64;
+    const size_t i = (size_t)(r0 + rr) * kH + n0 + nn;
+    const bf16 v = static_cast<bf16>(gs[rr] * (float)w[n0 + nn] * dsilu(Z[i]));
     dZ[i] = v;
-    dZT[(size_t)n * Mpad + r] = v;
+    tile[nn][rr] = v;
+  }
+  __syncthreads();
+  for (int q = tid; q < 64 * 64; q += 256) {
+    const int nn = q / 64, rr = q % 64;
+    dZT[(size_t)(n0 + nn) * Mpad + r0 + rr] = tile[nn][rr];
   }
 }
 
 // Column reduction for the head: gw[n] = sum_r dy[r] A[r,n]; gb = sum_r dy[r].
-__global__ void head_wgrad_kernel(const float *__restrict__ dy, const bf16 *__restrict__ Aact,
+// Reads the transposed activations AT[n][r] (contiguous in r).
+__global__ void head_wgrad_kernel(const float *__restrict__ dy, const bf16 *__restrict__ AT,
                                   int Mpad, float *gw, float *gb) {
   const int n = blockIdx.x;  // 0..kH (kH -> bias)
   float s = 0.f;
   for (int r = threadIdx.x; r < Mpad; r += blockDim.x)
-    s += dy[r] * (n < kH ? (float)Aact[(size_t)r * kH + n] : 1.f);
+    s += dy[r] * (n < kH ? (float)AT[(size_t)n * Mpad + r] : 1.f);
   __shared__ float red[256];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -513,11 +534,11 @@ static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const i
   FDADI_CHECK_LAUNCH();
   FDADI_RET(forward(m, Mpad, true, s));
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, m->loss, (int64_t)1);
-  hipLaunchKernelGGL(head_backward_kernel, dim3(Mpad), dim3(256), 0, s, m->y, d_T, d_idx, rows,
-                     Mpad, m->w4b, m->Z[kLayers - 1], m->dy, m->dZ, m->dZT, m->loss, m->tmean,
-                     m->tstd);
+  hipLaunchKernelGGL(head_backward_kernel, dim3(kH / 64, Mpad / 64), dim3(256), 0, s, m->y, d_T,
+                     d_idx, rows, Mpad, m->w4b, m->Z[kLayers - 1], m->dy, m->dZ, m->dZT, m->loss,
+                     m->tmean, m->tstd);
   hipLaunchKernelGGL(head_wgrad_kernel, dim3(kH + 1), dim3(256), 0, s, m->dy,
-                     m->Act[kLayers - 1], Mpad, m->g + m->off_w4, m->g + m->off_b4);
+                     m->ActT[kLayers - 1], Mpad, m->g + m->off_w4, m->g + m->off_b4);
   FDADI_CHECK_LAUNCH();
   bf16 *dz = m->dZ, *dzt = m->dZT, *dz2 = m->dZ2, *dzt2 = m->dZT2;
   for (int l = kLayers - 1; l >= 0; --l) {
