@@ -153,23 +153,35 @@ def surrogate_run(n_labels, n_queries, members, steps, seed=17):
     nval = max(len(y) // 10, 1)
     Pv, yv, Pt, yt = P[:nval], y[:nval], P[nval:], y[nval:]
     t = time.perf_counter()
-    ens, hists = S.train_ensemble(Pt, yt, members=members, steps=steps, log_every=steps // 4)
+    feller_min, max_resid = 1.5, 0.05
+    out["gate_feller_min"], out["train_max_resid"] = feller_min, max_resid
+    ens, hists = S.train_ensemble(Pt, yt, members=members, steps=steps, log_every=steps // 4,
+                                  max_resid=max_resid)
     fd_adi_amd._lib.sync()
     t_train = time.perf_counter() - t
     flop = 4.75e6 * 8192 * steps * members
     out["train_sec"] = round(t_train, 2)
     out["train_TFLOPs"] = round(flop / t_train / 1e12, 1)
     out["train_final_loss"] = [round(h[-1][1], 6) for h in hists]
+    out["baseline"] = "semi-analytic Heston price (device), network learns ADI - it"
+    base_v = S.cf_price(Pv)
+    out["baseline_only_val_rmse"] = float(np.sqrt(np.mean((base_v - yv) ** 2)))
+    out["baseline_only_val_max_abs_err"] = float(np.max(np.abs(base_v - yv)))
     mean_v, spread_v = ens.predict(Pv)
     out["val_rmse"] = float(np.sqrt(np.mean((mean_v - yv) ** 2)))
     out["val_max_abs_err"] = float(np.max(np.abs(mean_v - yv)))
-    tau, acc = S.calibrate_gate(ens, Pv, yv, 1e-4)
+    tau, acc = S.calibrate_gate(ens, Pv, yv, 1e-4, feller_min)
     out["gate_tau"] = tau
     out["gate_val_accept"] = round(acc, 4)
     # the same gate calibrated for looser tolerances (not used for serving)
-    out["val_accept_at_atol"] = {"%g" % a: round(S.calibrate_gate(ens, Pv, yv, a)[1], 4)
+    out["val_accept_at_atol"] = {"%g" % a: round(S.calibrate_gate(ens, Pv, yv, a,
+                                                                   feller_min)[1], 4)
                                  for a in (1e-3, 1e-2, 1e-1)}
-    pricer = S.SurrogatePricer(ens, tau)
+    reg = S.feller_ratio(Pv) >= feller_min
+    out["val_rmse_in_gate_region"] = float(np.sqrt(np.mean((mean_v[reg] - yv[reg]) ** 2)))
+    out["baseline_only_val_rmse_in_gate_region"] = float(
+        np.sqrt(np.mean((base_v[reg] - yv[reg]) ** 2)))
+    pricer = S.SurrogatePricer(ens, tau, feller_min=feller_min)
     Q = S.draw_params(n_queries, seed + 1, outside_frac=0.05)
     t = time.perf_counter()
     price, okq = pricer.price(Q)
@@ -181,6 +193,10 @@ def surrogate_run(n_labels, n_queries, members, steps, seed=17):
     _ = ens.predict(Q)
     fd_adi_amd._lib.sync()
     out["surrogate_only_options_per_s"] = round(n_queries / (time.perf_counter() - t), 1)
+    t = time.perf_counter()
+    _ = S.cf_price(Q)
+    fd_adi_amd._lib.sync()
+    out["semi_analytic_options_per_s"] = round(n_queries / (time.perf_counter() - t), 1)
     rng = np.random.default_rng(seed + 2)
     idx = rng.choice(n_queries, min(1024, n_queries), replace=False)
     truth = S.adi_prices(Q[idx])

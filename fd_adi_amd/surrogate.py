@@ -163,17 +163,36 @@ def lower_bound(P):
     return np.maximum(100.0 - 100.0 * P[:, 7] * np.exp(-P[:, 5] * P[:, 6]), 0.0)
 
 
+def cf_price(P):
+    """Semi-analytic Heston call price of feature rows P (S0 = 100), priced on
+    the device (fd_adi_amd.pricing): the networks learn the FP64 ADI price's
+    difference from it, i.e. the discretisation error of the label mesh."""
+    from .pricing import heston_call
+    P = np.asarray(P, dtype=float)
+    if P.shape[0] == 0:
+        return np.zeros(0)
+    return heston_call(100.0, 100.0 * P[:, 7], P[:, 5], P[:, 4], P[:, 6], P[:, 0], P[:, 1],
+                       P[:, 2], P[:, 3])
+
+
+# what the networks learn the label's difference from
+BASELINES = {"cf": cf_price, "bound": lower_bound}
+
+
 class Ensemble(object):
     """Independently initialised networks; the spread of their predictions
-    is the per-query error estimate.  Prices are lower_bound + network."""
+    is the per-query error estimate.  Prices are baseline + network, with the
+    baseline the semi-analytic price ("cf") or the call's lower bound
+    ("bound")."""
 
-    def __init__(self, members):
+    def __init__(self, members, base="cf"):
         self.members = list(members)
+        self.base = base
 
     def predict(self, P_host):
         Y = np.stack([m.predict(P_host) for m in self.members])
         spread = Y.std(axis=0, ddof=1) if len(self.members) > 1 else 0 * Y[0]
-        return lower_bound(P_host) + Y.mean(axis=0), spread
+        return BASELINES[self.base](P_host) + Y.mean(axis=0), spread
 
 
 def label_grids(P, nS=256, nv=128):
@@ -187,7 +206,7 @@ def label_grids(P, nS=256, nv=128):
     j = np.argmin(np.abs(S - 100.0), axis=1)
     # where K itself is the node nearest 100, put S0 on the neighbour on the
     # other side of 100 (monotonicity is kept: 100 lies between the two)
-    clash = S[rows, j] == K
+    clash = (S[rows, j] == K) & (K != 100.0)
     j = np.where(clash, np.where(K < 100.0, j + 1, j - 1), j)
     S[rows, j] = 100.0
     V = exponential_space_batch(0.0, P[:, 4], 5 * np.maximum(P[:, 1], P[:, 4]), 4.0, nv)
@@ -259,10 +278,28 @@ def draw_params(n, seed, outside_frac=0.0, widen=0.25):
     return P
 
 
-def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log_every=0):
+def feller_ratio(P):
+    """2 kappa theta / sigma^2 of feature rows P.  Below ~1.5 the variance
+    process crowds v = 0, where the label mesh resolves it poorly: there the
+    FP64 ADI labels differ from the exact price by 1e-2 and more, against
+    ~1e-3 elsewhere."""
+    P = np.asarray(P, dtype=float)
+    return 2.0 * P[:, 0] * P[:, 1] / P[:, 2] ** 2
+
+
+def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log_every=0,
+                   base="cf", max_resid=None):
     """Independently initialised networks trained on the same labels (each on
-    its own minibatch order); returns the Ensemble and the loss histories."""
-    t = np.asarray(y, dtype=float) - lower_bound(P)
+    its own minibatch order) to predict y - BASELINES[base](P); with
+    ``max_resid`` the rows whose target exceeds it in magnitude are left out
+    (labels far from the baseline come from regimes the label mesh does not
+    resolve; the gate sends those queries to ADI).  Returns the Ensemble and
+    the loss histories."""
+    P = np.asarray(P, dtype=float)
+    t = np.asarray(y, dtype=float) - BASELINES[base](P)
+    if max_resid is not None:
+        keep = np.abs(t) <= max_resid
+        P, t = P[keep], t[keep]
     mu, sd = float(np.mean(t)), float(np.std(t))
     nets, hists = [], []
     for k in range(members):
@@ -270,15 +307,20 @@ def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log
         hists.append(m.fit(P, t, steps, batch=batch, lr=lr, seed=seed + 7 * k,
                            log_every=log_every))
         nets.append(m)
-    return Ensemble(nets), hists
+    return Ensemble(nets, base), hists
 
 
-def calibrate_gate(ensemble, P, y, atol=1e-4):
+def calibrate_gate(ensemble, P, y, atol=1e-4, feller_min=0.0):
     """Largest spread threshold tau such that every validation query with
-    ensemble spread <= tau has |mean - y| <= atol (0 if none qualifies);
-    returns (tau, accepted fraction of the validation set)."""
+    ensemble spread <= tau (and Feller ratio >= ``feller_min``) has
+    |mean - y| <= atol (0 if none qualifies); returns (tau, accepted fraction
+    of the whole validation set)."""
     mean, spread = ensemble.predict(P)
     err = np.abs(mean - y)
+    region = feller_ratio(P) >= feller_min
+    n_all = float(len(y))
+    err, spread = err[region], spread[region]
+    y = np.asarray(y)[region]
     order = np.argsort(spread)
     bad = np.cumsum(err[order] > atol)
     ok = np.where(bad == 0)[0]
@@ -286,19 +328,20 @@ def calibrate_gate(ensemble, P, y, atol=1e-4):
         return 0.0, 0.0
     k = int(ok[-1])
     # stop below the next spread value, which would admit a failing query
-    return float(spread[order[k]]), (k + 1) / float(len(y))
+    return float(spread[order[k]]), (k + 1) / n_all
 
 
 class SurrogatePricer(object):
     """Ensemble surrogate with a per-query gate and the FP64 ADI fallback."""
 
-    def __init__(self, ensemble, tau, lo=BOX_LO, hi=BOX_HI):
+    def __init__(self, ensemble, tau, lo=BOX_LO, hi=BOX_HI, feller_min=0.0):
         self.ensemble, self.tau = ensemble, float(tau)
         self.lo, self.hi = np.asarray(lo, float), np.asarray(hi, float)
+        self.feller_min = float(feller_min)
 
     def gate(self, P, spread):
         inside = np.all((P >= self.lo) & (P <= self.hi), axis=1)
-        return inside & (spread <= self.tau)
+        return inside & (feller_ratio(P) >= self.feller_min) & (spread <= self.tau)
 
     def price(self, P, **adi_kw):
         P = np.asarray(P, dtype=float)

@@ -248,6 +248,23 @@ int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
 int fdadi_mlp_gemm_nt(const uint16_t *d_A, const uint16_t *d_B, int32_t M, int32_t N, int32_t K,
                       float *d_C, void *stream);
 
+/* ---- semi-analytic Heston call (csrc/cf.hip) ------------------------------------
+ * Replaces the reference's semi-analytic checker HestonCos
+ * (FiniteDifference/heston.py:490-693, used by
+ * FiniteDifference/tests/test_HestonOptionGPU.py:74-112) with a device pricer:
+ * Lewis's single-integral formula over the "little trap" characteristic
+ * function, Gauss-Legendre quadrature (512 nodes on [0, a], 64 on the tail).
+ * d_rows [n x 9] = S0, K, r, v0, T, kappa, theta, sigma, rho per option;
+ * d_out [n] call prices (NaN for S0, K, T, sigma <= 0, v0 or theta < 0, or
+ * |rho| >= 1).  Accurate to ~1e-9 for sigma >= 0.01; below sigma ~ 1e-3 the
+ * kappa theta / sigma^2 terms cancel and accuracy is lost.  Asynchronous on
+ * ``stream``; the node tables are copied to the current device on its first
+ * call. */
+int fdadi_heston_cf_call(const double *d_rows, int64_t n, double *d_out, void *stream);
+/* Gauss-Legendre nodes x[n] (ascending) and weights w[n] on (-1, 1), host only
+ * (the quadrature rule of fdadi_heston_cf_call). */
+int fdadi_gauss_legendre(int32_t n, double *x, double *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,8 +45,10 @@ def heston_call(S, K, r, v0, T, kappa, theta, sigma, rho):
 
     P = []
     for j in (1, 2):
-        val, _ = scipy.integrate.quad(integrand, 1e-10, 200.0, args=(j,),
-                                      limit=400, epsabs=1e-12, epsrel=1e-10)
+        # to infinity: for small (v0 + kappa theta T) / sigma the integrand
+        # still matters beyond u = 200 (1e-5 of the price at the box's edge)
+        val, _ = scipy.integrate.quad(integrand, 1e-10, np.inf, args=(j,),
+                                      limit=4000, epsabs=1e-13, epsrel=1e-11)
         P.append(0.5 + val / np.pi)
     return S * P[0] - K * np.exp(-r * T) * P[1]
 

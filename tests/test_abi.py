@@ -38,3 +38,14 @@ def test_error_channel_without_device():
     assert "null" in _lib.last_error() or "short" in _lib.last_error()
     st = _lib.lib.fdadi_array_binop(None, None, 10, 7, None)
     assert st == _lib.ERR_ARG
+
+
+def test_gauss_legendre_rule_matches_numpy():
+    """The host quadrature rule of the device Heston pricer (no GPU call)."""
+    import numpy as np
+    from fd_adi_amd.pricing import gauss_legendre
+    for n in (1, 2, 7, 64, 512):
+        x, w = gauss_legendre(n)
+        X, W = np.polynomial.legendre.leggauss(n)
+        np.testing.assert_allclose(x, X, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(w, W, rtol=0, atol=1e-14)
