@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from .SizedArrayPtr import SizedArrayPtr
 from .batch import HestonBatch
-from .utils import sinh_space_batch, exponential_space_batch, nearest_index
+from .utils import sinh_space_batch
 
 FEATURES = ("kappa", "theta", "sigma", "rho", "v0", "r", "T", "K/S0")
 BOX_LO = np.array([0.5, 0.01, 0.1, -0.9, 0.01, 0.0, 0.1, 0.7])
@@ -135,9 +135,14 @@ class MLP(object):
         self.predict_device(P, n, out)
         return out.to_numpy()
 
-    def fit(self, P_host, T_host, steps, batch=8192, lr=1e-3, seed=0, log_every=0):
+    def fit(self, P_host, T_host, steps, batch=8192, lr=1e-3, seed=0, log_every=0,
+            schedule="constant"):
         """Adam on (P, T) for ``steps`` minibatches drawn without replacement
-        per epoch; returns the loss history (standardised MSE)."""
+        per epoch, at rate ``lr`` ("constant") or decayed from ``lr`` to lr/100
+        along a half cosine ("cosine"); returns the loss history
+        (standardised MSE)."""
+        if schedule not in ("constant", "cosine"):
+            raise ValueError("schedule must be 'constant' or 'cosine'")
         P = SizedArrayPtr(np.ascontiguousarray(P_host, dtype=float))
         T = SizedArrayPtr(np.ascontiguousarray(T_host, dtype=float))
         n = T_host.shape[0]
@@ -149,7 +154,10 @@ class MLP(object):
         for k in range(steps):
             if pos == per_epoch:
                 perm, pos = DeviceInts(rng.permutation(n)), 0
-            self.train_step(P, T, perm, rows, lr, start=pos * rows)
+            lr_k = lr
+            if schedule == "cosine":
+                lr_k = lr * (0.01 + 0.99 * 0.5 * (1.0 + np.cos(np.pi * k / max(steps - 1, 1))))
+            self.train_step(P, T, perm, rows, lr_k, start=pos * rows)
             pos += 1
             if log_every and (k % log_every == 0 or k == steps - 1):
                 hist.append((k, self.last_loss()))
@@ -195,26 +203,60 @@ class Ensemble(object):
         return BASELINES[self.base](P_host) + Y.mean(axis=0), spread
 
 
+VAR_MAX = 1.0   # top of the label variance mesh (above 5x the box's largest variance)
+
+
 def label_grids(P, nS=256, nv=128):
-    """Meshes for feature rows P (vectorised): spot sinh mesh on [0, 5K]
-    clustered at K with K and S0 = 100 on nodes, variance power-law mesh on
-    [0, 5 max(theta, v0)] with v0 on a node."""
+    """Meshes for feature rows P (vectorised).  The spot mesh is K times one
+    unit sinh mesh on [0, 5] clustered at 1 (density 0.07, i.e. 7 at K = 100)
+    with K on a node; the variance mesh is the same power-law mesh
+    v_j = VAR_MAX (j / (nv - 1))^4 for every row.  Neither depends on S0 or
+    v0, which are not forced onto nodes: the label is interpolated there
+    (``label_price``).  Meshes that move with the parameters (nodes forced
+    onto S0 and v0, as the engine's own construction does) make the label
+    jump by ~1e-3 between neighbouring parameter values; these keep it
+    smooth, which is what a regressor needs."""
     P = np.asarray(P, dtype=float)
     K = 100.0 * P[:, 7]
-    S = sinh_space_batch(K, 5 * K, 7.0, nS)
-    rows = np.arange(S.shape[0])
-    j = np.argmin(np.abs(S - 100.0), axis=1)
-    # where K itself is the node nearest 100, put S0 on the neighbour on the
-    # other side of 100 (monotonicity is kept: 100 lies between the two)
-    clash = (S[rows, j] == K) & (K != 100.0)
-    j = np.where(clash, np.where(K < 100.0, j + 1, j - 1), j)
-    S[rows, j] = 100.0
-    V = exponential_space_batch(0.0, P[:, 4], 5 * np.maximum(P[:, 1], P[:, 4]), 4.0, nv)
-    return list(S), list(V)
+    unit = sinh_space_batch(np.array([1.0]), 5.0, 0.07, nS)[0]
+    v = VAR_MAX * (np.arange(nv) / float(nv - 1)) ** 4
+    return [k * unit for k in K], [v.copy() for _ in range(P.shape[0])]
+
+
+def _lagrange4(x, nodes):
+    """Weights [n, 4] of 4-point Lagrange interpolation at x [n] from the
+    node rows nodes [n, 4]."""
+    w = np.ones(nodes.shape)
+    for i in range(4):
+        for j in range(4):
+            if i != j:
+                w[:, i] *= (x - nodes[:, j]) / (nodes[:, i] - nodes[:, j])
+    return w
+
+
+def label_price(V, spots, vars_, S0, v0):
+    """Value of grids V [n, nS, nv] at (S0[n], v0[n]) by tensor-product cubic
+    Lagrange interpolation on the 4 x 4 nodes around the point (exact for
+    bicubic polynomials); meshes spots [n, nS], vars_ [n, nv]."""
+    V = np.asarray(V, dtype=float)
+    spots, vars_ = np.asarray(spots, dtype=float), np.asarray(vars_, dtype=float)
+    n, nS, nv = V.shape
+    S0 = np.broadcast_to(np.asarray(S0, dtype=float), (n,))
+    v0 = np.broadcast_to(np.asarray(v0, dtype=float), (n,))
+    rows = np.arange(n)
+    i = np.clip(np.array([np.searchsorted(spots[k], S0[k]) for k in range(n)]) - 2, 0, nS - 4)
+    j = np.clip(np.array([np.searchsorted(vars_[k], v0[k]) for k in range(n)]) - 2, 0, nv - 4)
+    di = i[:, None] + np.arange(4)[None, :]
+    dj = j[:, None] + np.arange(4)[None, :]
+    ws = _lagrange4(S0, spots[rows[:, None], di])
+    wv = _lagrange4(v0, vars_[rows[:, None], dj])
+    block = V[rows[:, None, None], di[:, :, None], dj[:, None, :]]   # [n, 4, 4]
+    return np.einsum("ni,nij,nj->n", ws, block, wv)
 
 
 def adi_prices(P, nS=256, nv=128, nt=200, scheme="cs", chunk=4096, grids=None):
-    """FP64 ADI prices (batched engine) for feature rows P."""
+    """FP64 ADI prices (batched engine) for feature rows P on the label
+    meshes, read at (S0 = 100, v0) by ``label_price``."""
     P = np.asarray(P, dtype=float)
     out = np.empty(P.shape[0])
     for c0 in range(0, P.shape[0], chunk):
@@ -229,9 +271,8 @@ def adi_prices(P, nS=256, nv=128, nt=200, scheme="cs", chunk=4096, grids=None):
                       for s, v, r in zip(spots, vars_, rows)])
         hb.configure(scheme, 0.5, 1.0)
         hb.step(nt)
-        V = hb.get_state()
-        for k, (s, v, r) in enumerate(zip(spots, vars_, rows)):
-            out[c0 + k] = V[k][nearest_index(s, 100.0), nearest_index(v, r[4])]
+        out[c0:c0 + len(rows)] = label_price(np.stack(hb.get_state()), np.stack(spots),
+                                             np.stack(vars_), 100.0, rows[:, 4])
     return out
 
 
@@ -288,7 +329,7 @@ def feller_ratio(P):
 
 
 def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log_every=0,
-                   base="cf", max_resid=None):
+                   base="cf", max_resid=None, schedule="constant"):
     """Independently initialised networks trained on the same labels (each on
     its own minibatch order) to predict y - BASELINES[base](P); with
     ``max_resid`` the rows whose target exceeds it in magnitude are left out
@@ -305,7 +346,7 @@ def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log
     for k in range(members):
         m = MLP(BOX_LO, BOX_HI, mu, sd, seed=seed + 101 * k, max_rows=batch)
         hists.append(m.fit(P, t, steps, batch=batch, lr=lr, seed=seed + 7 * k,
-                           log_every=log_every))
+                           log_every=log_every, schedule=schedule))
         nets.append(m)
     return Ensemble(nets, base), hists
 

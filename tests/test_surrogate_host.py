@@ -1,0 +1,46 @@
+"""Host logic of the surrogate's labels (CPU only): the label meshes and the
+interpolated read-out at (S0, v0)."""
+
+import numpy as np
+import numpy.testing as npt
+
+from fd_adi_amd.surrogate import (label_grids, label_price, draw_params, feller_ratio,
+                                  BOX_LO, BOX_HI, VAR_MAX)
+
+
+def test_label_meshes_scale_with_strike_and_ignore_v0():
+    P = draw_params(6, 4)
+    S, V = label_grids(P, 64, 32)
+    unit = S[0] / (100.0 * P[0, 7])
+    for s, v, p in zip(S, V, P):
+        K = 100.0 * p[7]
+        npt.assert_allclose(s / K, unit, rtol=1e-14, atol=1e-14)   # K x one unit mesh
+        assert np.any(s == K) and np.all(np.diff(s) > 0) and s[0] == 0.0
+        npt.assert_array_equal(v, V[0])                            # one variance mesh
+        assert v[0] == 0.0 and v[-1] == VAR_MAX and np.all(np.diff(v) > 0)
+
+
+def test_read_out_exact_for_bicubics():
+    P = draw_params(16, 5)
+    S, V = label_grids(P, 64, 32)
+
+    def f(s, v):
+        return 3.0 - 0.2 * s + 0.004 * s ** 2 - 1e-5 * s ** 3 + v * (2.0 + 0.01 * s) \
+            - 5.0 * v ** 2 + 9.0 * v ** 3 * (1.0 + 1e-4 * s ** 3)
+
+    G = np.stack([f(s[:, None], v[None, :]) for s, v in zip(S, V)])
+    got = label_price(G, np.stack(S), np.stack(V), 100.0, P[:, 4])
+    npt.assert_allclose(got, f(100.0, P[:, 4]), rtol=1e-12)
+
+
+def test_read_out_at_nodes_returns_the_node_value():
+    s = np.linspace(0.0, 10.0, 11)
+    v = np.linspace(0.0, 1.0, 6)
+    G = np.random.default_rng(0).random((1, 11, 6))
+    npt.assert_allclose(label_price(G, s[None], v[None], 4.0, 0.6), G[0, 4, 3], rtol=1e-13)
+
+
+def test_feller_ratio():
+    P = np.array([[2.0, 0.04, 0.4, -0.5, 0.04, 0.03, 1.0, 1.0]])
+    npt.assert_allclose(feller_ratio(P), [2 * 2.0 * 0.04 / 0.16])
+    assert np.all(BOX_LO < BOX_HI)
