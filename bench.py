@@ -312,6 +312,24 @@ def probe_mlp():
     log("probe-mlp: 50 training steps at batch 8192 done")
 
 
+def probe_cf():
+    """Fixed semi-analytic pricer dispatches for kernel traces (see --probe-cf)."""
+    import fd_adi_amd
+    from fd_adi_amd import surrogate as S
+    from fd_adi_amd.pricing import heston_call_device
+    from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
+    import numpy as np
+    P = S.draw_params(1 << 20, 3)
+    rows = np.ascontiguousarray(np.stack([np.full(len(P), 100.0), 100.0 * P[:, 7], P[:, 5],
+                                          P[:, 4], P[:, 6], P[:, 0], P[:, 1], P[:, 2],
+                                          P[:, 3]], axis=1))
+    d_rows, d_out = SizedArrayPtr(rows), SizedArrayPtr().alloc(len(P))
+    for _ in range(5):
+        heston_call_device(d_rows, len(P), d_out)
+    fd_adi_amd._lib.sync()
+    log("probe-cf: 5 launches pricing 2^20 options done")
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 summary."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_config1.txt")
@@ -344,6 +362,9 @@ def main():
     ap.add_argument("--probe-mlp", action="store_true",
                     help="profiling probe: 50 Adam steps of one surrogate network at batch "
                          "8192 on random features and targets; prints no JSON line")
+    ap.add_argument("--probe-cf", action="store_true",
+                    help="profiling probe: 5 launches of the semi-analytic Heston pricer on "
+                         "2^20 parameter rows; prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
@@ -358,6 +379,9 @@ def main():
         return
     if args.probe_mlp:
         probe_mlp()
+        return
+    if args.probe_cf:
+        probe_cf()
         return
     opt = config1()
     N = opt.spots.size * opt.vars.size
