@@ -33,3 +33,15 @@ def test_cli_matches_engine(gpu, tmp_path, capsys):
     assert len(files) == 1 and files[0].startswith("gpu_moneyness_")
     z = np.load(os.path.join(tmp_path, files[0]))
     assert z["V"].shape == (40, 20) and z["V"][e.idx] == price
+
+
+def test_bench_rejects_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with a message (no GPU use)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
