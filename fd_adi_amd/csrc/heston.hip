@@ -419,14 +419,46 @@ __device__ __forceinline__ SpotRow share_row_fields(double2 a, double2 b) {
 // at column jm2 = j - 2 on the last column only (the second-order backward
 // row of d/dv reaches two columns back; elsewhere out[3] = 0).  Offsets are
 // 32-bit unsigned (an option's grid has at most 2048 x 1024 nodes).
-__device__ __forceinline__ void load_row32(const double *__restrict__ f, unsigned rr, unsigned nv,
-                                           unsigned jm, unsigned j, unsigned jp, unsigned jm2,
-                                           bool last, double out[4]) {
+// Neighbouring columns of a tile: the value of lane jj - 1 / jj + 1 of the
+// quad (NL = 4, DPP quad_perm [0,0,1,2] / [1,2,3,3]) or of the DPP row
+// (NL = 16, row_shr:1 / row_shl:1).  Lanes 0 and NL - 1 get no neighbour
+// from these (they use their own edge load).
+template <int NL>
+__device__ __forceinline__ double from_prev_col(double x) {
+  constexpr int ctrl = NL == 4 ? 0x90 : 0x111;
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int NL>
+__device__ __forceinline__ double from_next_col(double x) {
+  constexpr int ctrl = NL == 4 ? 0xF9 : 0x101;
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// One state row as loaded: [0] the edge column jx (j - 1 on the tile's first
+// lane, j + 1 on its last, j elsewhere), [1] column j, [3] column j - 2 on
+// the grid's last column.  expand_row turns it into the window row
+// [j - 1, j, j + 1, j - 2] by taking the inner neighbours from the adjacent
+// lanes, which hold exactly those columns (a lane past the grid's last column
+// loads that column, which is also the clamped j + 1 of the last one).
+__device__ __forceinline__ void load_row_raw(const double *__restrict__ f, unsigned rr,
+                                             unsigned nv, unsigned jx, unsigned j, unsigned jm2,
+                                             bool last, double out[4]) {
   const unsigned b = rr * nv;
-  out[0] = f[b + jm];
+  out[0] = f[b + jx];
   out[1] = f[b + j];
-  out[2] = f[b + jp];
+  out[2] = 0.0;
   out[3] = last ? f[b + jm2] : 0.0;
+}
+template <int NL>
+__device__ __forceinline__ void expand_row(int jj, double w[4]) {
+  const double l = from_prev_col<NL>(w[1]);
+  const double r = from_next_col<NL>(w[1]);
+  w[2] = jj == NL - 1 ? w[0] : r;
+  w[0] = jj == 0 ? w[0] : l;
 }
 
 // A spot sweep: one work-group per (option, NL adjacent variance columns,
@@ -491,21 +523,28 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   const unsigned ujc = (unsigned)jc;
   const unsigned jm = jc > 0 ? ujc - 1 : 0u, jp = jc < nv - 1 ? ujc + 1 : ujc;
   const unsigned jm2 = jc > 1 ? ujc - 2 : 0u;
+  const unsigned jx = jj == 0 ? jm : (jj == NL - 1 ? jp : ujc);  // this lane's edge load
   const double tdt = o.tdt, dt = o.dt;
   const int dir_hi = o.barrier ? nS - 1 : -1;  // Dirichlet rows: 0 and dir_hi
 
   // Phase 1: walk the chunk with 3x3 windows, form the fused right-hand side
   // (and the next stage's explicit term), and run the local forward recurrence.
   double mu[MAXL], nreg[MAXL];
-  // rows i-1, i, i+1 and i+2 (in flight); [3] = column j-2 on the last column
+  // rows i-1, i, i+1 (expanded windows) and i+2 (in flight, as loaded);
+  // [3] = column j-2 on the last column
   double wu[4][4], wy[4][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
-    load_row32(u, clamp_row(s - 1 + r), unv, jm, ujc, jp, jm2, last, wu[r]);
+    load_row_raw(u, clamp_row(s - 1 + r), unv, jx, ujc, jm2, last, wu[r]);
   if (HAS_Y) {
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      load_row32(y2, clamp_row(s - 1 + r), unv, jm, ujc, jp, jm2, last, wy[r]);
+      load_row_raw(y2, clamp_row(s - 1 + r), unv, jx, ujc, jm2, last, wy[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    expand_row<NL>(jj, wu[r]);
+    if (HAS_Y) expand_row<NL>(jj, wy[r]);
   }
   double2 ta0, tb0, ta1, tb1;  // table rows i and i+1
   share_row_load<NL>(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
@@ -532,8 +571,11 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     const unsigned i2 = clamp_row(i + 2), i3 = clamp_row(i + 3);
     share_row_load<NL>(spr + i2 * kSpRow, jj, ta2, tb2);
     double nu[4], ny[4];
-    load_row32(u, i3, unv, jm, ujc, jp, jm2, last, nu);
-    if (HAS_Y) load_row32(y2, i3, unv, jm, ujc, jp, jm2, last, ny);
+    load_row_raw(u, i3, unv, jx, ujc, jm2, last, nu);
+    if (HAS_Y) load_row_raw(y2, i3, unv, jx, ujc, jm2, last, ny);
+    // row i+1 enters the window
+    expand_row<NL>(jj, wu[2]);
+    if (HAS_Y) expand_row<NL>(jj, wy[2]);
     const double uj2 = wu[1][3];
     const double yj2 = HAS_Y ? wy[1][3] : 0.0;
 
