@@ -117,3 +117,25 @@ def test_batched_labels_match_oracle(fd):
         rels.append(abs(y[k] - want) / abs(want))
     print("label rel errors: median %.2e max %.2e" % (np.median(rels), np.max(rels)))
     assert np.median(rels) < 1e-13
+
+
+def test_serving_gate_and_fallback(fd):
+    """SurrogatePricer: queries the gate accepts get baseline + ensemble mean,
+    the others (and every query outside the training box) get the FP64 ADI
+    price on the label meshes, exactly."""
+    from fd_adi_amd import surrogate as S
+    P = S.draw_params(2048, 21)
+    y = S.adi_prices(P, nS=48, nv=16, nt=40)
+    ok = S.within_call_bounds(P, y) & (np.abs(y - S.cf_price(P)) < 0.5)
+    ens, _ = S.train_ensemble(P[ok], y[ok], members=2, steps=60, batch=1024, max_resid=0.5)
+    Q = S.draw_params(256, 22, outside_frac=0.25)
+    mean, spread = ens.predict(Q)
+    tau = float(np.median(spread))          # about half the queries pass the spread test
+    pricer = S.SurrogatePricer(ens, tau)
+    price, accepted = pricer.price(Q, nS=48, nv=16, nt=40)
+    inside = np.all((Q >= S.BOX_LO) & (Q <= S.BOX_HI), axis=1)
+    npt.assert_array_equal(accepted, inside & (spread <= tau))
+    assert 0 < accepted.sum() < inside.sum()
+    npt.assert_array_equal(price[accepted], mean[accepted])
+    truth = S.adi_prices(Q[~accepted], nS=48, nv=16, nt=40)
+    npt.assert_array_equal(price[~accepted], truth)
