@@ -130,7 +130,7 @@ def batched(n_opt, seed, K, W):
     return n_opt * K / wall, wall / K, npts, shapes
 
 
-def surrogate_run(n_labels, n_queries, members, steps, seed=17):
+def surrogate_run(n_labels, n_queries, members, steps, seed=17, subsample=1.0):
     """Configs 3-4 at reduced scale: FP64 ADI labels from the batched engine
     (256 x 128, nt = 200, Craig-Sneyd), an ensemble of MLPs trained on the
     GPU, the spread gate calibrated on held-out labels for 1e-4 absolute
@@ -155,9 +155,10 @@ def surrogate_run(n_labels, n_queries, members, steps, seed=17):
     Pv, yv, Pt, yt = P[:nval], y[:nval], P[nval:], y[nval:]
     t = time.perf_counter()
     feller_min, max_resid = 0.0, 0.05
+    out["train_subsample"] = subsample
     out["gate_feller_min"], out["train_max_resid"] = feller_min, max_resid
     ens, hists = S.train_ensemble(Pt, yt, members=members, steps=steps, log_every=steps // 4,
-                                  max_resid=max_resid, schedule="cosine")
+                                  max_resid=max_resid, schedule="cosine", subsample=subsample)
     fd_adi_amd._lib.sync()
     t_train = time.perf_counter() - t
     flop = 4.75e6 * 8192 * steps * members
@@ -354,6 +355,8 @@ def main():
                     help="ADI-labelled training draws for the surrogate (0 = skip)")
     ap.add_argument("--surrogate-queries", type=int, default=1 << 15)
     ap.add_argument("--surrogate-steps", type=int, default=8000)
+    ap.add_argument("--surrogate-subsample", type=float, default=1.0,
+                    help="fraction of the labels each ensemble member trains on (bagging)")
     ap.add_argument("--probe", action="store_true",
                     help="profiling probe for rocprofv3 counter runs: 20 HV steps of the "
                          "2048x1024 option, then 5 launches of the scale kernel on 2^25 "
@@ -434,7 +437,7 @@ def main():
         log("extra: %s" % extra)
     if args.surrogate_labels:
         sur = surrogate_run(args.surrogate_labels, args.surrogate_queries, 4,
-                            args.surrogate_steps)
+                            args.surrogate_steps, subsample=args.surrogate_subsample)
         extra["surrogate"] = sur
         log("surrogate: %s" % sur)
 

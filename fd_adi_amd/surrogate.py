@@ -329,13 +329,15 @@ def feller_ratio(P):
 
 
 def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log_every=0,
-                   base="cf", max_resid=None, schedule="constant"):
+                   base="cf", max_resid=None, schedule="constant", subsample=1.0):
     """Independently initialised networks trained on the same labels (each on
     its own minibatch order) to predict y - BASELINES[base](P); with
     ``max_resid`` the rows whose target exceeds it in magnitude are left out
     (labels far from the baseline come from regimes the label mesh does not
-    resolve; the gate sends those queries to ADI).  Returns the Ensemble and
-    the loss histories."""
+    resolve; the gate sends those queries to ADI).  With ``subsample`` < 1
+    each member trains on its own random subset of that fraction of the rows
+    (bagging), so the members' spread also reflects how well the data pin
+    the function.  Returns the Ensemble and the loss histories."""
     P = np.asarray(P, dtype=float)
     t = np.asarray(y, dtype=float) - BASELINES[base](P)
     if max_resid is not None:
@@ -343,9 +345,16 @@ def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log
         P, t = P[keep], t[keep]
     mu, sd = float(np.mean(t)), float(np.std(t))
     nets, hists = [], []
+    if not 0.0 < subsample <= 1.0:
+        raise ValueError("subsample must be in (0, 1]")
     for k in range(members):
         m = MLP(BOX_LO, BOX_HI, mu, sd, seed=seed + 101 * k, max_rows=batch)
-        hists.append(m.fit(P, t, steps, batch=batch, lr=lr, seed=seed + 7 * k,
+        Pk, tk = P, t
+        if subsample < 1.0:
+            pick = np.random.default_rng(seed + 13 * k).permutation(len(t))
+            pick = np.sort(pick[:max(int(subsample * len(t)), 1)])
+            Pk, tk = P[pick], t[pick]
+        hists.append(m.fit(Pk, tk, steps, batch=batch, lr=lr, seed=seed + 7 * k,
                            log_every=log_every, schedule=schedule))
         nets.append(m)
     return Ensemble(nets, base), hists
