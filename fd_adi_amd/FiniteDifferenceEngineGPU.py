@@ -470,13 +470,15 @@ class HestonFiniteDifferenceEngine(FiniteDifferenceEngineADI):
             if getattr(option, "bottom", None):
                 raise NotImplementedError("Bottom barriers are not supported for GPU creation.")
         if grid:
-            self.spots = np.array(grid.mesh[0], dtype=float)
-            self.vars = np.array(grid.mesh[1], dtype=float)
+            # The grid's own mesh arrays: the S0 snap below moves grid.mesh[0]
+            # too (FiniteDifferenceEngineGPU.pyx:659-661).
+            self.spots = grid.mesh[0]
+            self.vars = grid.mesh[1]
         else:
             if vars is None:
                 vars = exponential_space(0.00, option.variance.value, var_max, varexp, nvols,
                                          force_exact=force_exact)
-            self.vars = np.array(vars, dtype=float)
+            self.vars = np.asarray(vars, dtype=float)
             if spots is None:
                 top = getattr(option, "top", None)
                 if top and not top[0]:
@@ -486,10 +488,17 @@ class HestonFiniteDifferenceEngine(FiniteDifferenceEngineADI):
                 else:
                     spots = sinh_space(option.strike - spot_min, spot_max - spot_min,
                                        spotdensity, nspots, force_exact=force_exact) + spot_min
-            self.spots = np.array(spots, dtype=float)
-            self.spots[np.argmin(np.abs(self.spots - option.spot))] = option.spot
             K = option.strike
-            grid = Grid([self.spots, self.vars], initializer=lambda *x: np.maximum(x[0] - K, 0))
+            # Payoff on the mesh as generated, before S0 is snapped onto it
+            # (FiniteDifferenceEngineGPU.pyx:682).
+            grid = Grid([np.asarray(spots, dtype=float), self.vars],
+                        initializer=lambda *x: np.maximum(x[0] - K, 0))
+            self.spots = grid.mesh[0]
+            self.vars = grid.mesh[1]
+        # S0 onto its nearest node, in place: self.spots is grid.mesh[0], so the
+        # grid's mesh moves with it while the payoff keeps the unsnapped node
+        # (FiniteDifferenceEngineGPU.pyx:685-686).
+        self.spots[np.argmin(np.abs(self.spots - option.spot))] = option.spot
         if getattr(option, "top", None) and not option.top[0]:
             self.barrier = option.top[1]
         self.grid = grid

@@ -96,6 +96,9 @@ _SIGS = {
     "fdadi_hb_create": (c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, _P(c_vp)]),
     "fdadi_hb_destroy": (c_int, [c_vp]),
     "fdadi_hb_configure": (c_int, [c_vp, c_int, c_double, c_double, c_vp]),
+    "fdadi_hb_update": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "fdadi_hb_call_payoff": (c_int, [c_vp, c_vp, c_vp]),
+    "fdadi_hb_read_bicubic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fdadi_hb_state": (c_int, [c_vp, _P(c_vp), _P(c_i64)]),
     "fdadi_hb_offsets": (c_int, [c_vp, c_vp]),
     "fdadi_hb_step": (c_int, [c_vp, c_i32, c_vp]),

@@ -14,6 +14,12 @@ import numpy as np
 from . import _lib
 from .SizedArrayPtr import SizedArrayPtr
 
+# fdadi_heston_option (include/fdadi.h) as a NumPy record
+_OPTION_DTYPE = np.dtype([("r", "f8"), ("kappa", "f8"), ("theta", "f8"), ("sigma", "f8"),
+                          ("rho", "f8"), ("dt", "f8"), ("nS", "i8"), ("nv", "i8"),
+                          ("barrier", "i4"), ("pad", "i4")])
+assert _OPTION_DTYPE.itemsize == ctypes.sizeof(_lib.HestonOptionStruct)
+
 SCHEMES = {
     "implicit": _lib.SCHEME_IMPLICIT, "i": _lib.SCHEME_IMPLICIT,
     "douglas": _lib.SCHEME_DOUGLAS, "d": _lib.SCHEME_DOUGLAS,
@@ -29,22 +35,12 @@ class HestonBatch(object):
     step ``dts[k]``."""
 
     def __init__(self, params, spots, vars, dts, stream=None):
-        n = len(params)
-        if not (len(spots) == len(vars) == len(dts) == n) or n == 0:
-            raise ValueError("params, spots, vars and dts must have the same nonzero length")
-        arr = (_lib.HestonOptionStruct * n)()
-        for k, p in enumerate(params):
-            a = arr[k]
-            a.r, a.kappa, a.theta = float(p["r"]), float(p["kappa"]), float(p["theta"])
-            a.sigma, a.rho, a.dt = float(p["sigma"]), float(p["rho"]), float(dts[k])
-            a.nS, a.nv = len(spots[k]), len(vars[k])
-            a.barrier = int(bool(p.get("barrier", 0)))
-        self.shapes = [(len(s), len(v)) for s, v in zip(spots, vars)]
-        sp = np.ascontiguousarray(np.concatenate([np.asarray(s, float) for s in spots]))
-        vp = np.ascontiguousarray(np.concatenate([np.asarray(v, float) for v in vars]))
+        arr, sp, vp = self._pack(params, spots, vars, dts)
+        n = len(arr)
+        self.shapes = [(int(a), int(b)) for a, b in zip(arr["nS"], arr["nv"])]
         self.stream = stream if stream is not None else _lib.default_stream()
         h = ctypes.c_void_p()
-        _lib.call("fdadi_hb_create", n, ctypes.cast(arr, ctypes.c_void_p), _lib.ptr(sp),
+        _lib.call("fdadi_hb_create", n, arr.ctypes.data_as(ctypes.c_void_p), _lib.ptr(sp),
                   _lib.ptr(vp), self.stream, ctypes.byref(h))
         self._h = h
         self.n = n
@@ -56,6 +52,55 @@ class HestonBatch(object):
         self._state = p.value
         self.total = tot.value
         self.scheme = None
+
+    @staticmethod
+    def _pack(params, spots, vars, dts):
+        """fdadi_heston_option records and the packed meshes.  ``params`` is a
+        list of dicts or one dict of per-option arrays; ``spots``/``vars`` are
+        lists of meshes or 2-D arrays with one mesh per row."""
+        n = len(dts)
+        if isinstance(params, dict):
+            cols = {k: np.broadcast_to(np.asarray(v, float), (n,)) for k, v in params.items()}
+        else:
+            if len(params) != n:
+                raise ValueError("params and dts must have the same length")
+            cols = {k: np.array([float(p.get(k, 0.0)) for p in params])
+                    for k in ("r", "kappa", "theta", "sigma", "rho", "barrier")}
+        if not (len(spots) == len(vars) == n) or n == 0:
+            raise ValueError("params, spots, vars and dts must have the same nonzero length")
+        arr = np.zeros(n, dtype=_OPTION_DTYPE)
+        for k in ("r", "kappa", "theta", "sigma", "rho"):
+            arr[k] = cols[k]
+        arr["dt"] = np.asarray(dts, float)
+        arr["barrier"] = (np.asarray(cols.get("barrier", np.zeros(n))) != 0).astype(np.int32)
+        if isinstance(spots, np.ndarray) and spots.ndim == 2:
+            arr["nS"], sp = spots.shape[1], np.ascontiguousarray(spots, dtype=float).ravel()
+        else:
+            arr["nS"] = [len(s) for s in spots]
+            sp = np.ascontiguousarray(np.concatenate([np.asarray(s, float) for s in spots]))
+        if isinstance(vars, np.ndarray) and vars.ndim == 2:
+            arr["nv"], vp = vars.shape[1], np.ascontiguousarray(vars, dtype=float).ravel()
+        else:
+            arr["nv"] = [len(v) for v in vars]
+            vp = np.ascontiguousarray(np.concatenate([np.asarray(v, float) for v in vars]))
+        return arr, sp, vp
+
+    def update(self, params, spots, vars, dts):
+        """New parameters, meshes and steps for the same options (same count
+        and grid shapes); a configured scheme and a captured graph are kept."""
+        arr, sp, vp = self._pack(params, spots, vars, dts)
+        _lib.call("fdadi_hb_update", self._h, len(arr), arr.ctypes.data_as(ctypes.c_void_p),
+                  _lib.ptr(sp), _lib.ptr(vp), self.stream)
+
+    def call_payoff(self, strikes):
+        """Initial state max(S - K_k, 0) on every option's grid (``strikes``:
+        a device array of n strikes)."""
+        _lib.call("fdadi_hb_call_payoff", self._h, strikes.p, self.stream)
+
+    def read_bicubic(self, S0, v0, out):
+        """Every option's state read at (S0[k], v0[k]) by bicubic Lagrange
+        interpolation into ``out`` (device arrays of n values)."""
+        _lib.call("fdadi_hb_read_bicubic", self._h, S0.p, v0.p, out.p, self.stream)
 
     def __del__(self):
         try:

@@ -1042,6 +1042,69 @@ __global__ void spot_factor_kernel(Ctx c, const Tile *__restrict__ tiles, int nt
   }
 }
 
+// One work-group per spot row of every option: V(i, j) = max(S_i - K_k, 0).
+__global__ void call_payoff_kernel(const HOpt *opts, const int64_t *row_opt,
+                                   const double *__restrict__ spots, int64_t sp_n,
+                                   const double *__restrict__ strikes, double *state) {
+  const int64_t g = blockIdx.x;
+  if (g >= sp_n) return;
+  const int k = (int)row_opt[g];
+  const HOpt o = opts[k];
+  const int64_t i = g - o.so;
+  const double v = fmax(spots[g] - strikes[k], 0.0);
+  double *row = state + o.off + i * o.nv;
+  for (int64_t j = threadIdx.x; j < o.nv; j += blockDim.x) row[j] = v;
+}
+
+// First node of the 4-point stencil around x on mesh m[0..n): nodes
+// i-2 .. i+1 with i the first node >= x, shifted to stay inside the mesh.
+__device__ __forceinline__ int64_t stencil4(const double *m, int64_t n, double x) {
+  int64_t lo = 0, hi = n;  // lower bound: first index with m[i] >= x
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (m[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  int64_t s = lo - 2;
+  if (s > n - 4) s = n - 4;
+  if (s < 0) s = 0;
+  return s;
+}
+
+__device__ __forceinline__ void lagrange4w(double x, const double *nd, double w[4]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double p = 1.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (a != b) p *= (x - nd[b]) / (nd[a] - nd[b]);
+    w[a] = p;
+  }
+}
+
+// One thread per option: the state read at (S0_k, v0_k) by tensor-product
+// cubic Lagrange interpolation on the 4 x 4 nodes around the point.
+__global__ void read_bicubic_kernel(const HOpt *opts, int nopt, const double *__restrict__ spots,
+                                    const double *__restrict__ vars,
+                                    const double *__restrict__ state, const double *S0,
+                                    const double *v0, double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nopt) return;
+  const HOpt o = opts[k];
+  const double *ms = spots + o.so, *mv = vars + o.vo;
+  const int64_t i = stencil4(ms, o.nS, S0[k]), j = stencil4(mv, o.nv, v0[k]);
+  double ws[4], wv[4];
+  lagrange4w(S0[k], ms + i, ws);
+  lagrange4w(v0[k], mv + j, wv);
+  const double *V = state + o.off;
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double *row = V + (i + a) * o.nv + j;
+    acc += ws[a] * (((row[0] * wv[0] + row[1] * wv[1]) + row[2] * wv[2]) + row[3] * wv[3]);
+  }
+  out[k] = acc;
+}
+
 }  // namespace
 }  // namespace fdadi
 
@@ -1314,6 +1377,41 @@ int step_once(fdadi_hb *hb, hipStream_t s, StageEvents *tm = nullptr) {
   return FDADI_ERR_STATE;
 }
 
+// Spot and variance tables from the meshes and parameters on the device.
+int build_tables(fdadi_hb *hb, hipStream_t s) {
+  hipLaunchKernelGGL(spot_tables_kernel, dim3(grid_for(hb->sp_n, 256)), dim3(256), 0, s,
+                     hb->d_opts, hb->nopt, hb->d_sp_row, hb->d_spots, hb->d_sp, hb->sp_n,
+                     hb->d_spr);
+  FDADI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(var_tables_kernel, dim3(grid_for(hb->vt_n, 256)), dim3(256), 0, s,
+                     hb->d_opts, hb->d_vt_row, hb->d_vars, hb->d_vt, hb->vt_n);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+// Step sizes of the configured scheme and the time-invariant factorisations
+// of I - tdt A0 and I - tdt A1, on the device; synchronises.
+int refactor(fdadi_hb *hb, hipStream_t s) {
+  for (int k = 0; k < hb->nopt; ++k) {
+    HOpt &o = hb->h_opts[k];
+    o.dt = hb->base_dt[k] * hb->dt_scale;
+    o.tdt = hb->scheme == FDADI_SCHEME_IMPLICIT ? o.dt : hb->theta * o.dt;
+  }
+  FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
+                               hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(var_factor_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0, s, hb->d_opts,
+                     hb->nopt, hb->d_vars, hb->d_vt, hb->vt_n);
+  FDADI_CHECK_LAUNCH();
+  hipLaunchKernelGGL(spot_factor_kernel, dim3(hb->nftiles), dim3(64), 0, s, ctx_of(hb),
+                     hb->d_ftiles, hb->nftiles, hb->d_rd0);
+  FDADI_CHECK_LAUNCH();
+  // Keep the host copy in step with the factor constants written on the device.
+  FDADI_TRY_HIP(hipMemcpyAsync(hb->h_opts.data(), hb->d_opts, sizeof(HOpt) * hb->nopt,
+                               hipMemcpyDeviceToHost, s));
+  FDADI_TRY_HIP(hipStreamSynchronize(s));
+  return FDADI_OK;
+}
+
 void free_hb(fdadi_hb *hb) {
   if (!hb) return;
   if (hb->gexec) (void)hipGraphExecDestroy(hb->gexec);
@@ -1432,13 +1530,7 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   up(hb->d_stiles, st.data(), sizeof(Tile) * st.size());
   up(hb->d_ftiles, ft.data(), sizeof(Tile) * ft.size());
   if (e == hipSuccess && build_var_tiles(hb) != FDADI_OK) e = hipErrorUnknown;
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(spot_tables_kernel, dim3(grid_for(so, 256)), dim3(256), 0, s,
-                       hb->d_opts, nopt, hb->d_sp_row, hb->d_spots, hb->d_sp, so, hb->d_spr);
-    hipLaunchKernelGGL(var_tables_kernel, dim3(grid_for(vo, 256)), dim3(256), 0, s, hb->d_opts,
-                       hb->d_vt_row, hb->d_vars, hb->d_vt, vo);
-    e = hipGetLastError();
-  }
+  if (e == hipSuccess && build_tables(hb, s) != FDADI_OK) e = hipErrorUnknown;
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     free_hb(hb);
@@ -1466,23 +1558,7 @@ int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale, 
   hb->scheme = scheme;
   hb->theta = theta;
   hb->dt_scale = dt_scale;
-  for (int k = 0; k < hb->nopt; ++k) {
-    HOpt &o = hb->h_opts[k];
-    o.dt = hb->base_dt[k] * dt_scale;
-    o.tdt = scheme == FDADI_SCHEME_IMPLICIT ? o.dt : theta * o.dt;
-  }
-  FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
-                               hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(var_factor_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0, s, hb->d_opts,
-                     hb->nopt, hb->d_vars, hb->d_vt, hb->vt_n);
-  FDADI_CHECK_LAUNCH();
-  hipLaunchKernelGGL(spot_factor_kernel, dim3(hb->nftiles), dim3(64), 0, s, ctx_of(hb),
-                     hb->d_ftiles, hb->nftiles, hb->d_rd0);
-  FDADI_CHECK_LAUNCH();
-  // Keep the host copy in step with the factor constants written on the device.
-  FDADI_TRY_HIP(hipMemcpyAsync(hb->h_opts.data(), hb->d_opts, sizeof(HOpt) * hb->nopt,
-                               hipMemcpyDeviceToHost, s));
-  FDADI_TRY_HIP(hipStreamSynchronize(s));
+  FDADI_RET(refactor(hb, s));
   if (!keep_graph) {
     if (hb->gexec) {
       (void)hipGraphExecDestroy(hb->gexec);
@@ -1493,6 +1569,60 @@ int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale, 
       hb->graph = nullptr;
     }
   }
+  return FDADI_OK;
+}
+
+int fdadi_hb_update(fdadi_hb *hb, int32_t nopt, const fdadi_heston_option *opts,
+                    const double *spots, const double *vars, void *stream) {
+  FDADI_REQUIRE(hb && opts && spots && vars, FDADI_ERR_ARG, "fdadi_hb_update: bad arguments");
+  FDADI_REQUIRE(nopt == hb->nopt, FDADI_ERR_ARG,
+                "fdadi_hb_update: option count differs from the engine's");
+  for (int k = 0; k < nopt; ++k)
+    FDADI_REQUIRE(opts[k].nS == hb->h_opts[k].nS && opts[k].nv == hb->h_opts[k].nv,
+                  FDADI_ERR_ARG,
+                  "fdadi_hb_update: option " + std::to_string(k) + ": grid shape differs");
+  hipStream_t s = as_stream(stream);
+  for (int k = 0; k < nopt; ++k) {
+    HOpt &o = hb->h_opts[k];
+    const fdadi_heston_option &q = opts[k];
+    o.r = q.r;
+    o.kappa = q.kappa;
+    o.theta = q.theta;
+    o.sigma = q.sigma;
+    o.rho = q.rho;
+    o.dt = q.dt;
+    o.barrier = q.barrier ? 1 : 0;
+    hb->base_dt[k] = q.dt;
+  }
+  // The host arrays may be pageable: synchronous copies, after the stream's
+  // earlier work (which may still read the tables) has finished.
+  FDADI_TRY_HIP(hipStreamSynchronize(s));
+  FDADI_TRY_HIP(hipMemcpy(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * nopt,
+                          hipMemcpyHostToDevice));
+  FDADI_TRY_HIP(hipMemcpy(hb->d_spots, spots, sizeof(double) * hb->sp_n, hipMemcpyHostToDevice));
+  FDADI_TRY_HIP(hipMemcpy(hb->d_vars, vars, sizeof(double) * hb->vt_n, hipMemcpyHostToDevice));
+  FDADI_RET(build_tables(hb, s));
+  if (hb->scheme >= 0) FDADI_RET(refactor(hb, s));
+  FDADI_TRY_HIP(hipStreamSynchronize(s));
+  return FDADI_OK;
+}
+
+int fdadi_hb_call_payoff(fdadi_hb *hb, const double *d_strikes, void *stream) {
+  FDADI_REQUIRE(hb && d_strikes, FDADI_ERR_ARG, "fdadi_hb_call_payoff: bad arguments");
+  hipLaunchKernelGGL(call_payoff_kernel, dim3(hb->sp_n), dim3(128), 0, as_stream(stream),
+                     hb->d_opts, hb->d_sp_row, hb->d_spots, hb->sp_n, d_strikes, hb->d_state);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+int fdadi_hb_read_bicubic(const fdadi_hb *hb, const double *d_S0, const double *d_v0,
+                          double *d_out, void *stream) {
+  FDADI_REQUIRE(hb && d_S0 && d_v0 && d_out, FDADI_ERR_ARG,
+                "fdadi_hb_read_bicubic: bad arguments");
+  hipLaunchKernelGGL(read_bicubic_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0,
+                     as_stream(stream), hb->d_opts, hb->nopt, hb->d_spots, hb->d_vars, hb->d_state,
+                     d_S0, d_v0, d_out);
+  FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
 

@@ -6,15 +6,18 @@ result file naming), for the GPU engine only:
 
     python -m fd_adi_amd.price_one_option -nx 150 150 -nt 150 --gpu --scheme hv
 
+The result file is the reference's: ``[grid.mesh, grid.domain[-1]]`` pickled
+with the highest protocol into ``<name>.txt`` (price_one_option.py:193,
+199-208), written to ``--out-dir`` (default: the current directory).
 Differences: ``--cpu`` and ``--mc`` (the reference's CPU engine and Monte
-Carlo) are not part of this package and are rejected; the result file is a
-NumPy ``.npz`` (``spots``, ``vars``, ``V``) instead of a pickle, written to
-``--out-dir`` (default: the current directory); ``--scheme`` also accepts
-``cs`` and ``mcs``.
+Carlo) are not part of this package and are rejected; ``--npz`` writes a
+NumPy ``.npz`` (``spots``, ``vars``, ``V``) instead of the pickle;
+``--scheme`` also accepts ``cs`` and ``mcs``.
 """
 
 import argparse
 import os
+import pickle
 import sys
 
 import numpy as np
@@ -45,6 +48,8 @@ def read_args(argv=None):
     p.add_argument("--cpu", action="store_true")
     p.add_argument("--mc", metavar="int", type=int, default=0)
     p.add_argument("--out-dir", default=".")
+    p.add_argument("--npz", action="store_true",
+                   help="write spots, vars and V as .npz instead of the pickled list")
     opt = p.parse_args(argv)
     if opt.top is not None and not opt.barrier:
         p.error("--barrier required with --top")
@@ -77,13 +82,15 @@ def new_engine(opt):
 
 
 def filestring(opt, engine):
-    """Result file name of price_one_option.py:166-193 (extension .npz)."""
+    """Result file name of price_one_option.py:166-193 (``.txt``, or ``.npz``
+    with --npz)."""
     idxs, idxv = engine.idx
     parts = ["gpu", "moneyness", opt.spot / opt.strike, "tenor", opt.tenor,
              "ir", opt.interest_rate, "var", opt.variance, "volofvar", opt.vol_of_var,
              "corr", opt.correlation, "rev", opt.mean_reversion, "nspot", opt.nx[0],
              "nvols", opt.nx[1], "idxs", idxs, "idxv", idxv, "ntime", opt.nt]
-    return os.path.join(opt.out_dir, "_".join(map(str, parts)) + ".npz")
+    ext = ".npz" if getattr(opt, "npz", False) else ".txt"
+    return os.path.join(opt.out_dir, "_".join(map(str, parts)) + ext)
 
 
 def run(opt):
@@ -104,7 +111,11 @@ def main(argv=None):
     e, price = run(opt)
     print("FD:", price)
     fn = filestring(opt, e)
-    np.savez(fn, spots=e.spots, vars=e.vars, V=e.grid.domain[-1])
+    if opt.npz:
+        np.savez(fn, spots=e.spots, vars=e.vars, V=e.grid.domain[-1])
+    else:
+        with open(fn, "wb") as f:
+            pickle.dump([e.grid.mesh, e.grid.domain[-1]], f, pickle.HIGHEST_PROTOCOL)
     if opt.verbose:
         print("wrote", fn)
     return 0

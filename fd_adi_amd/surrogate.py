@@ -207,20 +207,29 @@ VAR_MAX = 1.0   # top of the label variance mesh (above 5x the box's largest var
 
 
 def label_grids(P, nS=256, nv=128):
-    """Meshes for feature rows P (vectorised).  The spot mesh is K times one
-    unit sinh mesh on [0, 5] clustered at 1 (density 0.07, i.e. 7 at K = 100)
-    with K on a node; the variance mesh is the same power-law mesh
-    v_j = VAR_MAX (j / (nv - 1))^4 for every row.  Neither depends on S0 or
-    v0, which are not forced onto nodes: the label is interpolated there
-    (``label_price``).  Meshes that move with the parameters (nodes forced
-    onto S0 and v0, as the engine's own construction does) make the label
-    jump by ~1e-3 between neighbouring parameter values; these keep it
-    smooth, which is what a regressor needs."""
+    """Meshes for feature rows P (vectorised): arrays [n, nS] and [n, nv].
+    The spot mesh is K times one unit sinh mesh on [0, 5] clustered at 1
+    (density 0.07, i.e. 7 at K = 100) with K on a node; the variance mesh is
+    the same power-law mesh v_j = VAR_MAX (j / (nv - 1))^4 for every row.
+    Neither depends on S0 or v0, which are not forced onto nodes: the label
+    is interpolated there (``label_price``).  Meshes that move with the
+    parameters (nodes forced onto S0 and v0, as the engine's own construction
+    does) make the label jump by ~1e-3 between neighbouring parameter values;
+    these keep it smooth, which is what a regressor needs."""
     P = np.asarray(P, dtype=float)
     K = 100.0 * P[:, 7]
-    unit = sinh_space_batch(np.array([1.0]), 5.0, 0.07, nS)[0]
+    unit = _unit_spot_mesh(nS)
     v = VAR_MAX * (np.arange(nv) / float(nv - 1)) ** 4
-    return [k * unit for k in K], [v.copy() for _ in range(P.shape[0])]
+    return K[:, None] * unit[None, :], np.broadcast_to(v, (P.shape[0], nv)).copy()
+
+
+_UNIT_MESHES = {}
+
+
+def _unit_spot_mesh(nS):
+    if nS not in _UNIT_MESHES:
+        _UNIT_MESHES[nS] = sinh_space_batch(np.array([1.0]), 5.0, 0.07, nS)[0]
+    return _UNIT_MESHES[nS]
 
 
 def _lagrange4(x, nodes):
@@ -235,9 +244,10 @@ def _lagrange4(x, nodes):
 
 
 def label_price(V, spots, vars_, S0, v0):
-    """Value of grids V [n, nS, nv] at (S0[n], v0[n]) by tensor-product cubic
-    Lagrange interpolation on the 4 x 4 nodes around the point (exact for
-    bicubic polynomials); meshes spots [n, nS], vars_ [n, nv]."""
+    """Host read-out of grids V [n, nS, nv] at (S0[n], v0[n]) by
+    tensor-product cubic Lagrange interpolation on the 4 x 4 nodes around the
+    point (exact for bicubic polynomials); meshes spots [n, nS], vars_ [n, nv].
+    The labeller does the same on the device (fdadi_hb_read_bicubic)."""
     V = np.asarray(V, dtype=float)
     spots, vars_ = np.asarray(spots, dtype=float), np.asarray(vars_, dtype=float)
     n, nS, nv = V.shape
@@ -254,26 +264,110 @@ def label_price(V, spots, vars_, S0, v0):
     return np.einsum("ni,nij,nj->n", ws, block, wv)
 
 
-def adi_prices(P, nS=256, nv=128, nt=200, scheme="cs", chunk=4096, grids=None):
-    """FP64 ADI prices (batched engine) for feature rows P on the label
-    meshes, read at (S0 = 100, v0) by ``label_price``."""
-    P = np.asarray(P, dtype=float)
-    out = np.empty(P.shape[0])
-    for c0 in range(0, P.shape[0], chunk):
-        rows = P[c0:c0 + chunk]
-        if grids is None:
-            spots, vars_ = label_grids(rows, nS, nv)
-        else:
-            spots, vars_ = grids[0][c0:c0 + chunk], grids[1][c0:c0 + chunk]
-        params = [dict(kappa=r[0], theta=r[1], sigma=r[2], rho=r[3], r=r[5]) for r in rows]
-        hb = HestonBatch(params, spots, vars_, [r[6] / nt for r in rows])
-        hb.set_state([np.maximum(s[:, None] - 100.0 * r[7], 0.0) + 0 * v[None, :]
-                      for s, v, r in zip(spots, vars_, rows)])
-        hb.configure(scheme, 0.5, 1.0)
-        hb.step(nt)
-        out[c0:c0 + len(rows)] = label_price(np.stack(hb.get_state()), np.stack(spots),
-                                             np.stack(vars_), 100.0, rows[:, 4])
-    return out
+def _h2d(dev, host):
+    host = np.ascontiguousarray(host, dtype=float)
+    _lib.call("fdadi_array_h2d", dev.p, _lib.ptr(host), host.size, _lib.default_stream())
+
+
+class AdiLabeler(object):
+    """FP64 ADI prices of feature rows on the label meshes, computed on the
+    device in batches.
+
+    One ``HestonBatch`` per capacity tier (``tiers``: options per batch) and
+    its captured graph of ``nt`` steps serve batch after batch: the rows'
+    parameters and meshes are loaded with ``HestonBatch.update`` (the graph
+    stays valid), the call payoff is written on the device, the graph is
+    replayed and the bicubic read-out at (S0 = 100, v0) runs on the device;
+    only the prices come back.  A batch shorter than its tier repeats its
+    last row (the extra options are discarded).  With ``tune`` the launch
+    shapes of each engine are picked by ``HestonBatch.autotune``."""
+
+    def __init__(self, nS=256, nv=128, nt=200, scheme="cs", theta=0.5, tiers=(256, 1024, 4096),
+                 tune=True):
+        self.nS, self.nv, self.nt = int(nS), int(nv), int(nt)
+        self.scheme, self.theta = scheme, float(theta)
+        self.tiers = tuple(sorted(int(t) for t in tiers))
+        self.tune = tune
+        self._engines = {}
+
+    def _engine(self, cap):
+        if cap not in self._engines:
+            mid = 0.5 * (BOX_LO + BOX_HI)
+            rows = np.repeat(mid[None, :], cap, axis=0)
+            hb = HestonBatch(*self._batch_args(rows))
+            hb.configure(self.scheme, self.theta, 1.0)
+            bufs = [SizedArrayPtr().alloc(cap) for _ in range(4)]   # K, S0, v0, out
+            _h2d(bufs[0], 100.0 * rows[:, 7])
+            hb.call_payoff(bufs[0])
+            if self.tune:
+                hb.autotune(steps=1)
+            hb.capture(self.nt)
+            self._engines[cap] = (hb, bufs)
+        return self._engines[cap]
+
+    def _batch_args(self, rows):
+        spots, vars_ = label_grids(rows, self.nS, self.nv)
+        params = dict(kappa=rows[:, 0], theta=rows[:, 1], sigma=rows[:, 2], rho=rows[:, 3],
+                      r=rows[:, 5])
+        return params, spots, vars_, rows[:, 6] / self.nt
+
+    def _tier(self, m):
+        for t in self.tiers:
+            if m <= t:
+                return t
+        return self.tiers[-1]
+
+    def price(self, P):
+        """FP64 ADI price of every feature row of P (S0 = 100)."""
+        P = np.asarray(P, dtype=float)
+        out = np.empty(P.shape[0])
+        c0 = 0
+        while c0 < P.shape[0]:
+            cap = self._tier(P.shape[0] - c0)
+            rows = P[c0:c0 + cap]
+            m = rows.shape[0]
+            if m < cap:
+                rows = np.vstack([rows, np.repeat(rows[-1:], cap - m, axis=0)])
+            hb, (K, S0, v0, res) = self._engine(cap)
+            hb.update(*self._batch_args(rows))
+            _h2d(K, 100.0 * rows[:, 7])
+            _h2d(S0, np.full(cap, 100.0))
+            _h2d(v0, rows[:, 4])
+            hb.call_payoff(K)
+            hb.replay()
+            hb.read_bicubic(S0, v0, res)
+            out[c0:c0 + m] = res.to_numpy()[:m]
+            c0 += m
+        return out
+
+    def grids(self, P):
+        """Value grids [n, nS, nv] of feature rows P (one tier-sized batch at
+        most; for tests)."""
+        P = np.asarray(P, dtype=float)
+        cap = self._tier(P.shape[0])
+        if P.shape[0] > cap:
+            raise ValueError("grids: at most %d rows" % cap)
+        m = P.shape[0]
+        rows = np.vstack([P, np.repeat(P[-1:], cap - m, axis=0)]) if m < cap else P
+        hb, (K, S0, v0, res) = self._engine(cap)
+        hb.update(*self._batch_args(rows))
+        _h2d(K, 100.0 * rows[:, 7])
+        hb.call_payoff(K)
+        hb.replay()
+        return np.stack(hb.get_state()[:m])
+
+
+_LABELERS = {}
+
+
+def adi_prices(P, nS=256, nv=128, nt=200, scheme="cs"):
+    """FP64 ADI prices (batched engine on the device) for feature rows P on
+    the label meshes, read at (S0 = 100, v0); one ``AdiLabeler`` per grid
+    shape, kept for the process."""
+    key = (int(nS), int(nv), int(nt), scheme)
+    if key not in _LABELERS:
+        _LABELERS[key] = AdiLabeler(nS, nv, nt, scheme)
+    return _LABELERS[key].price(P)
 
 
 def within_call_bounds(P, y, slack=1.0):
@@ -360,44 +454,78 @@ def train_ensemble(P, y, members=4, steps=2000, batch=8192, lr=1e-3, seed=0, log
     return Ensemble(nets, base), hists
 
 
-def calibrate_gate(ensemble, P, y, atol=1e-4, feller_min=0.0):
-    """Largest spread threshold tau such that every validation query with
-    ensemble spread <= tau (and Feller ratio >= ``feller_min``) has
-    |mean - y| <= atol (0 if none qualifies); returns (tau, accepted fraction
-    of the whole validation set)."""
+def calibrate_threshold(score, err, atol=1e-4, max_fail=0.0):
+    """Largest threshold tau on the per-query error estimate ``score`` such
+    that, among calibration queries with score <= tau, the fraction whose
+    actual error ``err`` exceeds ``atol`` is at most ``max_fail`` (0: none).
+    Ties are respected: tau always sits at a score value below which every
+    query with that value is admitted.  Returns (tau, accepted fraction of
+    the calibration set); tau = -inf (admit nothing) when no threshold
+    qualifies."""
+    score = np.asarray(score, dtype=float)
+    err = np.asarray(err, dtype=float)
+    if score.size == 0:
+        return -np.inf, 0.0
+    order = np.argsort(score, kind="stable")
+    s, bad = score[order], np.cumsum(err[order] > atol)
+    n = np.arange(1, s.size + 1)
+    ok = bad <= max_fail * n
+    # a cut after position k is valid only where the next score is larger
+    ok[:-1] &= s[1:] > s[:-1]
+    ok &= np.isfinite(s)
+    idx = np.nonzero(ok)[0]
+    if idx.size == 0:
+        return -np.inf, 0.0
+    k = int(idx[-1])
+    return float(s[k]), (k + 1) / float(s.size)
+
+
+def calibrate_gate(ensemble, P, y, atol=1e-4, feller_min=0.0, max_fail=0.0):
+    """Spread threshold of an ensemble (at least two members) calibrated on
+    held-out rows P with FP64 ADI labels y (``calibrate_threshold``); rows
+    with Feller ratio below ``feller_min`` are never admitted.  Returns (tau,
+    accepted fraction of all the rows)."""
+    if len(ensemble.members) < 2:
+        raise ValueError("a spread gate needs an ensemble of at least two networks")
     mean, spread = ensemble.predict(P)
-    err = np.abs(mean - y)
+    err = np.abs(mean - np.asarray(y, dtype=float))
     region = feller_ratio(P) >= feller_min
-    n_all = float(len(y))
-    err, spread = err[region], spread[region]
-    y = np.asarray(y)[region]
-    order = np.argsort(spread)
-    bad = np.cumsum(err[order] > atol)
-    ok = np.where(bad == 0)[0]
-    if ok.size == 0:
-        return 0.0, 0.0
-    k = int(ok[-1])
-    # stop below the next spread value, which would admit a failing query
-    return float(spread[order[k]]), (k + 1) / n_all
+    tau, frac = calibrate_threshold(spread[region], err[region], atol, max_fail)
+    return tau, frac * region.mean()
 
 
 class SurrogatePricer(object):
-    """Ensemble surrogate with a per-query gate and the FP64 ADI fallback."""
+    """Ensemble surrogate with a per-query gate and the FP64 ADI fallback.
 
-    def __init__(self, ensemble, tau, lo=BOX_LO, hi=BOX_HI, feller_min=0.0):
+    A query is answered by the surrogate (semi-analytic price + ensemble
+    mean) when it lies inside the training box, its Feller ratio is at least
+    ``feller_min`` and its error estimate is at most ``tau``; every other
+    query goes to ``adi`` (default: ``adi_prices`` on the label meshes).  The
+    error estimate is the ensemble spread, or ``error_model(P)`` when given
+    (a callable returning one estimate per row)."""
+
+    def __init__(self, ensemble, tau, lo=BOX_LO, hi=BOX_HI, feller_min=0.0, error_model=None,
+                 adi=None):
+        if error_model is None and len(ensemble.members) < 2:
+            raise ValueError("a spread gate needs an ensemble of at least two networks")
         self.ensemble, self.tau = ensemble, float(tau)
         self.lo, self.hi = np.asarray(lo, float), np.asarray(hi, float)
         self.feller_min = float(feller_min)
+        self.error_model = error_model
+        self.adi = adi if adi is not None else adi_prices
 
-    def gate(self, P, spread):
+    def score(self, P, spread):
+        return spread if self.error_model is None else self.error_model(P)
+
+    def gate(self, P, score):
         inside = np.all((P >= self.lo) & (P <= self.hi), axis=1)
-        return inside & (feller_ratio(P) >= self.feller_min) & (spread <= self.tau)
+        return inside & (feller_ratio(P) >= self.feller_min) & (score <= self.tau)
 
-    def price(self, P, **adi_kw):
+    def price(self, P):
         P = np.asarray(P, dtype=float)
         mean, spread = self.ensemble.predict(P)
-        ok = self.gate(P, spread)
+        ok = self.gate(P, self.score(P, spread))
         out = mean.copy()
         if (~ok).any():
-            out[~ok] = adi_prices(P[~ok], **adi_kw)
+            out[~ok] = self.adi(P[~ok])
         return out, ok

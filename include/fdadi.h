@@ -192,6 +192,27 @@ int fdadi_hb_destroy(fdadi_hb *hb);
  * (theta and dt are read from device memory at every step). */
 int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale,
                        void *stream);
+/* New parameters, step sizes and meshes for the engine's options, of the
+ * same count and grid shapes as at creation (host arrays as for create); the
+ * tables are rebuilt and, when configured, the factorisations too, so a
+ * captured graph stays valid.  This is how one engine (and one graph) serves
+ * batch after batch of the surrogate's fallback queries; the reference builds
+ * a new engine per option (FiniteDifferenceEngineGPU.pyx:603-694, driven per
+ * option by run_parameters.py:16-36).  Synchronises. */
+int fdadi_hb_update(fdadi_hb *hb, int32_t nopt, const fdadi_heston_option *opts,
+                    const double *spots, const double *vars, void *stream);
+/* European call payoff max(S - K_k, 0) on every option's grid (d_strikes:
+ * device array of nopt strikes); the initial condition of
+ * FiniteDifferenceEngineGPU.pyx:682. */
+int fdadi_hb_call_payoff(fdadi_hb *hb, const double *d_strikes, void *stream);
+/* Each option's state read at (S0_k, v0_k) (device arrays of nopt) by
+ * tensor-product cubic Lagrange interpolation on the 4 x 4 nodes around the
+ * point (stencil: nodes i-2 .. i+1 along each axis, i the first node >= the
+ * coordinate, shifted inside the mesh) into d_out[nopt].  The reference
+ * reads the value at a node instead (FiniteDifferenceEngineGPU.pyx:727-738);
+ * the surrogate's labels need S0 and v0 off the nodes. */
+int fdadi_hb_read_bicubic(const fdadi_hb *hb, const double *d_S0, const double *d_v0,
+                          double *d_out, void *stream);
 /* State of option k lives at d_state + offset[k] (layout (nS,nv) row-major). */
 int fdadi_hb_state(fdadi_hb *hb, double **d_state, int64_t *total);
 int fdadi_hb_offsets(const fdadi_hb *hb, int64_t *offsets);

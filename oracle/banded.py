@@ -248,6 +248,9 @@ class BandedOperator(object):
             ret = self.fold_vector(ret, unfold=True)
         if self.R is not None:
             ret = ret + self.R
+        c = getattr(self, "post_scale", None)
+        if c is not None:
+            ret = c * ret
         return back(ret)
 
     def solve(self, V):
@@ -262,8 +265,11 @@ class BandedOperator(object):
             rhs = rhs - self.R
         if self.is_folded():
             rhs = self.fold_vector(rhs)
-        if getattr(self, "solver", "lapack") == "thomas" and self.solve_banded_offsets == (1, 1):
+        solver = getattr(self, "solver", "lapack")
+        if solver == "thomas" and self.solve_banded_offsets == (1, 1):
             ret = self._thomas(rhs)
+        elif solver == "chunked" and self.solve_banded_offsets == (1, 1):
+            ret = self._chunked(rhs)
         else:
             ret = spl.solve_banded(self.solve_banded_offsets, self.data, rhs)
         return back(ret)
@@ -308,6 +314,57 @@ class BandedOperator(object):
         x[:, L - 1] = dp[:, L - 1]
         for i in range(L - 2, -1, -1):
             x[:, i] = dp[:, i] - cp[:, i] * x[:, i + 1]
+        return x.ravel()
+
+    def _chunked(self, rhs, chunk=16):
+        """The same elimination (no pivoting) associated the way a parallel
+        line solve does it -- another rounding-order probe, like _thomas:
+        pivots rd_i = 1 / (b_i - (a_i rd_{i-1}) c_{i-1}); the forward sweep in
+        scaled form y_i = rd_i d_i + mu_i y_{i-1} (mu_i = -a_i rd_i) and the
+        backward sweep x_i = y_i + n_i x_{i+1} (n_i = -c_i rd_i), each split
+        into chunks of ``chunk`` rows: a chunk's affine map y -> E + M y is
+        formed with a zero carry, the maps are composed chunk to chunk to give
+        each chunk its incoming carry, and the chunk is run again from it."""
+        nb, L = self.blocks, self.block_len
+        c = self.data[0].reshape(nb, L)      # c[:, i] multiplies x[i] in row i-1
+        b = self.data[1].reshape(nb, L)
+        a = self.data[2].reshape(nb, L)      # a[:, i] multiplies x[i] in row i+1
+        d = rhs.reshape(nb, L)
+        sub = np.zeros((nb, L))
+        sub[:, 1:] = a[:, :-1]
+        sup = np.zeros((nb, L))
+        sup[:, :-1] = c[:, 1:]
+        rd = np.empty((nb, L))
+        prev_rd = np.zeros(nb)
+        prev_c = np.zeros(nb)
+        for i in range(L):
+            rd[:, i] = 1.0 / (b[:, i] - (sub[:, i] * prev_rd) * prev_c)
+            prev_rd, prev_c = rd[:, i], sup[:, i]
+        mu = -sub * rd
+        nn = -sup * rd
+        ds = rd * d
+
+        def sweep(m, f, order):
+            starts = list(range(0, L, chunk))
+            carry = np.zeros(nb)
+            out = np.empty((nb, L))
+            for s0 in (starts if order > 0 else starts[::-1]):
+                rows = list(range(s0, min(s0 + chunk, L)))
+                if order < 0:
+                    rows = rows[::-1]
+                E, M = np.zeros(nb), np.ones(nb)
+                for i in rows:
+                    E = f[:, i] + m[:, i] * E
+                    M = m[:, i] * M
+                y = carry
+                for i in rows:
+                    y = f[:, i] + m[:, i] * y
+                    out[:, i] = y
+                carry = E + M * carry
+            return out
+
+        y = sweep(mu, ds, +1)
+        x = sweep(nn, y, -1)
         return x.ravel()
 
     # -- boundaries --------------------------------------------------------------

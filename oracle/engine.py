@@ -19,6 +19,16 @@ of the reference's GPU engine, which is what the product replaces):
   The reference's order is Python 2 dict order (arbitrary); the difference
   between the two orders is the oracle's own rounding noise floor, which the
   parity tests use to calibrate the grid tolerance.
+* ``vmax_row`` -- "reference" (default): the d2/dv2 row at v = vmax is the
+  reference's free second-order backward stencil, which reaches two nodes
+  back (heston.py:350-351, the -2 band the engines fold away);
+  "known_derivative": the tridiagonal row of a known first derivative 0
+  there, 2 (V[n-2] - V[n-1]) / h^2.  Not the reference's operator: a probe
+  that shows what the reference's row does to stability on meshes whose
+  first spacing is tiny (tests/test_oracle_stability.py).
+* ``scale_late`` -- the explicit operators dt A and theta dt A applied as
+  dt (A x + R) instead of (dt A) x + dt R: the same arithmetic associated
+  differently, a rounding-order probe (tests/parity.py).
 * ``implicit_residual`` -- the CPU implicit scheme keeps the boundary residual
   R in (I - dt A) x = V + dt R (FiniteDifferenceEngine.pyx:631); the GPU
   engine disables it (FiniteDifferenceEngineGPU.pyx:187-188).  Default True
@@ -56,7 +66,7 @@ class HestonOracle(object):
 
     def __init__(self, params, spots, vars, r_split_float32=False,
                  implicit_residual=True, firsts_reversed=False, tri_solver="lapack",
-                 apply_order=None):
+                 apply_order=None, vmax_row="reference", scale_late=False):
         self.p = params
         self.spots = np.array(spots, dtype=float)
         self.vars = np.array(vars, dtype=float)
@@ -69,6 +79,10 @@ class HestonOracle(object):
         self.tri_solver = tri_solver
         # "reversed": every operator apply sums its diagonals in reverse order
         self.apply_order = apply_order
+        if vmax_row not in ("reference", "known_derivative"):
+            raise ValueError("vmax_row must be 'reference' or 'known_derivative'")
+        self.vmax_row = vmax_row
+        self.scale_late = bool(scale_late)
         self.t = 0.0
         self.simple_operators = {}
         self.operators = {}
@@ -124,6 +138,8 @@ class HestonOracle(object):
         if self.p.barrier:
             b[(0,)] = (b[(0,)][0], (0, 0.0))
             b[(0, 0)] = b[(0,)]
+        if self.vmax_row == "known_derivative":
+            b[(1, 1)] = (b[(1, 1)][0], (None, 0.0))
         return b
 
     @staticmethod
@@ -190,6 +206,16 @@ class HestonOracle(object):
     def _dims(self):
         return [k for k in (0, 1) if k in self.operators]
 
+    def _scaled(self, op, c):
+        """c * op: the operator scaled (the reference), or -- with
+        ``scale_late`` -- applied unscaled and its result scaled, a
+        rounding-order probe: c A x + c R instead of (c A) x + c R."""
+        if not self.scale_late:
+            return op * c
+        B = op.copy()
+        B.post_scale = c
+        return B
+
     def _firsts(self, dt):
         # The reference sums the explicit operators in Python 2 dict order
         # (FiniteDifference/FiniteDifferenceEngine.pyx:958), which is arbitrary;
@@ -197,14 +223,14 @@ class HestonOracle(object):
         keys = list(self.operators)
         if self.firsts_reversed:
             keys = keys[::-1]
-        return [self._ordered(self.operators[k] * dt) for k in keys]
+        return [self._ordered(self._scaled(self.operators[k], dt)) for k in keys]
 
     def _ordered(self, op):
         op.apply_order = self.apply_order
         return op
 
     def _les_lis(self, dt, theta, fold):
-        Les = [self._ordered(self.operators[k] * (theta * dt)) for k in self._dims()]
+        Les = [self._ordered(self._scaled(self.operators[k], theta * dt)) for k in self._dims()]
         Lis = [(self.operators[k] * (theta * -dt)).add(1, inplace=True)
                for k in self._dims()]
         for L in Les + Lis:

@@ -2,10 +2,12 @@
 
 Each fixture holds the inputs of one solve (parameters, meshes, payoff, steps,
 time step, theta, scheme) and the oracle's value grid for it, together with
-the same oracle run with its explicit operators summed in the opposite order
-(``V_alt``: the reference sums them in Python 2 dict order, so the difference
-is the rounding floor a parity test must allow) and the semi-analytic Heston
-call price at (S0, v0).
+``floors``: how far the oracle's grid moves (max abs) when only the order of
+its floating-point operations changes -- explicit operators summed in the
+opposite order (the reference sums them in Python 2 dict order), band
+products summed in reverse, Thomas elimination instead of LAPACK -- which is
+the rounding floor a parity test must allow (tests/parity.py), and the
+semi-analytic Heston call price at (S0, v0).
 
 The oracle is the NumPy/SciPy restatement of the reference CPU engine
 (FiniteDifference/FiniteDifferenceEngine.pyx:616-1016); the meshes come from
@@ -26,6 +28,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from oracle.engine import HestonOracle, HestonParams  # noqa: E402
 from oracle.heston_cf import heston_call  # noqa: E402
+from tests.parity import PROBES  # noqa: E402
 from fd_adi_amd.utils import sinh_space, exponential_space  # noqa: E402
 
 # name: (nS, nv, scheme, n, dt, theta, parameters, spot_max, var_max, barrier)
@@ -80,15 +83,15 @@ def make(name):
     O = HestonOracle(hp, spots, vars_)
     U0 = O.payoff()
     V = solve(O, scheme, n, dt, theta, U0)
-    V_alt = solve(HestonOracle(hp, spots, vars_, firsts_reversed=True), scheme, n, dt, theta,
-                  U0)
+    floors = np.array([np.max(np.abs(V - solve(HestonOracle(hp, spots, vars_, **kw), scheme, n,
+                                                 dt, theta, U0))) for kw in PROBES])
     cf = np.nan if barrier is not None else heston_call(
         p["spot"], p["strike"], p["r"], p["v0"], p["T"], p["kappa"], p["theta"], p["sigma"],
         p["rho"])
     meta = dict(name=name, scheme=scheme, n=n, dt=dt, theta=theta, params=p,
                 spot_max=spot_max, var_max=var_max, barrier=barrier, idx=list(map(int, O.idx)))
     return dict(meta=np.array(json.dumps(meta)), spots=spots, vars=vars_, U0=U0, V=V,
-                V_alt=V_alt, price_cf=np.array(cf))
+                floors=floors, price_cf=np.array(cf))
 
 
 def main():

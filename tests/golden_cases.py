@@ -21,8 +21,9 @@ class Case(object):
         self.p = self.meta["params"]
         self.barrier = self.meta["barrier"]
         self.idx = tuple(self.meta["idx"])
-        for k in ("spots", "vars", "U0", "V", "V_alt"):
+        for k in ("spots", "vars", "U0", "V", "floors"):
             setattr(self, k, z[k])
+        self.floor = float(np.max(self.floors))
         self.price_cf = float(z["price_cf"])
 
 

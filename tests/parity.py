@@ -1,14 +1,20 @@
 """Parity criteria shared by the GPU tests.
 
 Price: relative error at the (S0, v0) node <= 1e-12 (strict).
-Grid: max |V_gpu - V_oracle| <= max(1e-10, 10 * floor), where ``floor`` is
-the oracle's own rounding noise: the difference between the oracle and the
-same oracle summing its explicit operators in the opposite order (the
-reference's order is Python 2 dict order, i.e. arbitrary).  Far-field rows
-(S near Smax, values ~1e3, coefficients ~1e4) make that floor exceed 1e-10
-on the reference's own grids, so a fixed 1e-10 would reject the oracle
-against itself.
+Grid: max |V_gpu - V_oracle| <= max(1e-10, 4 * floor), where ``floor`` is
+the oracle's own rounding noise: the largest change of the oracle's grid
+when only the order of its floating-point operations changes -- its explicit
+operators summed in the opposite order (the reference's order is Python 2
+dict order, i.e. arbitrary), every band product summed in reverse, or the
+tridiagonal systems eliminated by Thomas's algorithm instead of LAPACK.
+Far-field rows (S near Smax, values ~1e3, coefficients ~1e4) make that floor
+exceed 1e-10 on the reference's own grids, so a fixed 1e-10 would reject the
+oracle against itself.  Measured on MI355X over every parity case of the
+suite, the GPU's error is at most 2.3x that floor.
 """
+
+import json
+import os
 
 import numpy as np
 
@@ -16,24 +22,56 @@ from oracle.engine import HestonOracle
 
 PRICE_RTOL = 1e-12
 GRID_ATOL = 1e-10
-FLOOR_FACTOR = 10.0
+FLOOR_FACTOR = 4.0
+
+# oracle switches that change only the order of floating-point operations
+PROBES = (dict(firsts_reversed=True), dict(apply_order="reversed"), dict(tri_solver="thomas"))
+
+
+def alternates(O):
+    """The oracle ``O`` under each rounding-order probe."""
+    return [HestonOracle(O.p, O.spots, O.vars, **kw) for kw in PROBES]
 
 
 def reordered(O):
+    """The oracle with its explicit sum reversed (one probe)."""
     return HestonOracle(O.p, O.spots, O.vars, firsts_reversed=True)
 
 
-def grid_tolerance(W, W_alt):
-    floor = float(np.max(np.abs(W - W_alt)))
+def floor_of(W, alts):
+    """Largest |W - W_probe| over the probe grids ``alts`` (a list of grids,
+    one grid, or an already computed floor)."""
+    if np.ndim(alts) == 0:
+        return float(alts)
+    if isinstance(alts, np.ndarray) and alts.ndim == np.ndim(W):
+        alts = [alts]
+    return max(float(np.max(np.abs(W - a))) for a in alts)
+
+
+def grid_tolerance(W, alts):
+    floor = floor_of(W, alts)
     return max(GRID_ATOL, FLOOR_FACTOR * floor), floor
 
 
-def check(V, W, W_alt, idx):
-    tol, floor = grid_tolerance(W, W_alt)
+def record(**fields):
+    """Append one parity measurement (JSON line) to $FDADI_PARITY_LOG when it
+    is set; the per-case error table is built from these lines."""
+    path = os.environ.get("FDADI_PARITY_LOG")
+    if not path:
+        return
+    fields["test"] = os.environ.get("PYTEST_CURRENT_TEST", "").split(" ")[0]
+    with open(path, "a") as f:
+        f.write(json.dumps(fields) + "\n")
+
+
+def check(V, W, alts, idx):
+    tol, floor = grid_tolerance(W, alts)
     err = float(np.max(np.abs(V - W)))
     rel = abs(V[idx] - W[idx]) / abs(W[idx])
     msg = "grid err %.3e (tol %.3e, oracle floor %.3e), price rel %.3e" % (err, tol, floor, rel)
     print(msg)
+    record(err=err, floor=floor, tol=tol, ratio=err / floor if floor > 0 else None,
+           price_rel=float(rel), max_abs_V=float(np.max(np.abs(W))))
     assert err <= tol, msg
     assert rel <= PRICE_RTOL, msg
     return err, rel, floor

@@ -2,6 +2,7 @@
 small solve on the GPU)."""
 
 import os
+import pickle
 
 import numpy as np
 import pytest
@@ -31,8 +32,19 @@ def test_cli_matches_engine(gpu, tmp_path, capsys):
     assert price == p2
     files = os.listdir(tmp_path)
     assert len(files) == 1 and files[0].startswith("gpu_moneyness_")
-    z = np.load(os.path.join(tmp_path, files[0]))
-    assert z["V"].shape == (40, 20) and z["V"][e.idx] == price
+    assert files[0].endswith(".txt")
+    # the reference's file: [grid.mesh, grid.domain[-1]], pickled (written above)
+    with open(os.path.join(tmp_path, files[0]), "rb") as f:
+        mesh, V = pickle.load(f)
+    assert V.shape == (40, 20) and V[e.idx] == price
+    # the strike (99) is not S0: S0 is snapped onto the mesh in place; v0 is
+    # a node to 10 decimals (the reference's check, price_one_option.py:130-133)
+    assert mesh[0][e.idx[0]] == 100.0 and abs(mesh[1][e.idx[1]] - 0.04) < 1e-10
+    np.testing.assert_array_equal(mesh[0], e.grid.mesh[0])
+    assert cli.main(args + ["--npz"]) == 0
+    z = [f for f in os.listdir(tmp_path) if f.endswith(".npz")]
+    assert len(z) == 1
+    np.testing.assert_array_equal(np.load(os.path.join(tmp_path, z[0]))["V"], V)
 
 
 def test_bench_rejects_more_than_one_gpu():

@@ -44,6 +44,18 @@ def test_oracle_reproduces_fixture(name):
     npt.assert_allclose(V, c.V, rtol=0, atol=1e-11)
 
 
+@pytest.mark.parametrize("name", [n for n in NAMES if not n.startswith("config0")])
+def test_oracle_reproduces_floors(name):
+    """The stored rounding-order floors come back from the probes."""
+    from tests.parity import PROBES
+    c = load(name)
+    V = solve(HestonOracle(params(c), c.spots, c.vars), c, c.U0)
+    got = [np.max(np.abs(V - solve(HestonOracle(params(c), c.spots, c.vars, **kw), c, c.U0)))
+           for kw in PROBES]
+    npt.assert_allclose(got, c.floors, rtol=1e-6, atol=1e-14)
+    assert np.max(c.floors) > 0
+
+
 @pytest.mark.parametrize("name", NAMES)
 def test_meshes_regenerate(name):
     c = load(name)

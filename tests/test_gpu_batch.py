@@ -9,7 +9,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 from oracle.engine import HestonOracle, HestonParams
-from tests.parity import grid_tolerance, reordered, PRICE_RTOL
+from tests.parity import grid_tolerance, alternates, record, PRICE_RTOL, FLOOR_FACTOR
 from fd_adi_amd.utils import sinh_space, exponential_space
 
 
@@ -49,11 +49,14 @@ def test_batch_uneven_grids_match_oracle(fd, scheme):
     got = hb.get_state()
     for k, o in enumerate(O):
         want = getattr(o, "solve_" + scheme)(20, dts[k], o.payoff(), theta=0.5)
-        alt = getattr(reordered(o), "solve_" + scheme)(20, dts[k], o.payoff(), theta=0.5)
+        alt = [getattr(a, "solve_" + scheme)(20, dts[k], o.payoff(), theta=0.5)
+               for a in alternates(o)]
         tol, floor = grid_tolerance(want, alt)
         err = np.max(np.abs(got[k] - want))
         rel = abs(got[k][o.idx] - want[o.idx]) / abs(want[o.idx])
         print(k, got[k].shape, "err %.3e tol %.3e floor %.3e price rel %.2e" % (err, tol, floor, rel))
+        record(err=float(err), floor=floor, tol=tol, ratio=float(err) / floor if floor else None,
+               price_rel=float(rel), case="option %d %s" % (k, got[k].shape))
         assert err <= tol, (k, err, tol)
         assert rel <= PRICE_RTOL, (k, rel)
 
@@ -100,27 +103,37 @@ def test_launch_shapes_agree(fd):
 
 
 def test_full_size_config1_steps(fd):
-    """2048 x 1024, rho = -0.7, HV: two steps against the oracle, then the
-    Dirichlet row after 20 steps."""
+    """2048 x 1024, rho = -0.7, HV: the first 50 steps against the oracle
+    (the oracle and its three rounding-order probes run in parallel CPU
+    processes), then the Dirichlet row after 20 more."""
+    from tests.oracle_pool import solve_many
+    from tests.parity import PROBES
     spots = sinh_space(100, 500, 7.0, 2048)
     vars_ = exponential_space(0, 0.04, 0.2, 4.0, 1024)
     spots[np.argmin(np.abs(spots - 100))] = 100
     P = [dict(r=0.06, kappa=1.0, theta=0.04, sigma=0.2, rho=-0.7)]
-    O = HestonOracle(HestonParams(spot=100, strike=100, r=0.06, v0=0.04, T=1.0, kappa=1.0,
-                                  theta=0.04, sigma=0.2, rho=-0.7), spots, vars_)
-    hb = fd.HestonBatch(P, [spots], [vars_], [1 / 2000.0])
+    pf = dict(spot=100, strike=100, r=0.06, v0=0.04, T=1.0, kappa=1.0, theta=0.04, sigma=0.2,
+              rho=-0.7)
+    O = HestonOracle(HestonParams(**pf), spots, vars_)
+    n, dt = 50, 1 / 2000.0
+    hb = fd.HestonBatch(P, [spots], [vars_], [dt])
     U0 = O.payoff()
     hb.set_state([U0])
     hb.configure("hv", 0.5)
-    hb.step(2)
+    hb.step(n)
     got = hb.get_state()[0]
-    want = O.solve_hundsdorferverwer(2, 1 / 2000.0, U0)
-    alt = reordered(O).solve_hundsdorferverwer(2, 1 / 2000.0, U0)
+    res = solve_many([(pf, spots, vars_, dict(kw), "solve_hundsdorferverwer", (n, dt, U0, 0.5))
+                      for kw in ({},) + PROBES])
+    want, alt = res[0], res[1:]
     tol, floor = grid_tolerance(want, alt)
     err = np.max(np.abs(got - want))
-    print("2048x1024 HV 2 steps: err %.3e tol %.3e floor %.3e" % (err, tol, floor))
-    assert err <= tol
-    hb.step(18)
+    rel = abs(got[O.idx] - want[O.idx]) / abs(want[O.idx])
+    print("2048x1024 HV %d steps: err %.3e tol %.3e floor %.3e price rel %.2e"
+          % (n, err, tol, floor, rel))
+    record(err=float(err), floor=floor, tol=tol, ratio=float(err) / floor if floor else None,
+           price_rel=float(rel), case="2048x1024 HV %d steps" % n)
+    assert err <= tol and rel <= PRICE_RTOL
+    hb.step(20)
     V = hb.get_state()[0]
     assert np.all(V[0] == 0) and np.all(np.isfinite(V))
 
@@ -180,8 +193,9 @@ def test_config2_batch_subset_matches_oracle(fd):
     ~1e-8 on the grid (and ~1e-12 relative at the price node) when only the
     order of its own explicit sum, of its band products, or of its
     tridiagonal elimination changes.  The grid bar is therefore
-    max(1e-10, 10 x the largest of those three moves); the price node is
-    held to the same absolute bar (its relative error is printed)."""
+    max(1e-10, 4 x the largest of those three moves); the price node is
+    held to the same absolute bar (its relative error is printed and
+    recorded)."""
     from fd_adi_amd.workloads import batch_draws
     opts = batch_draws(4096, 2024)
     hb = fd.HestonBatch([o.params for o in opts], [o.spots for o in opts],
@@ -203,10 +217,12 @@ def test_config2_batch_subset_matches_oracle(fd):
         floor = max(np.max(np.abs(want - solve(firsts_reversed=True))),
                     np.max(np.abs(want - solve(apply_order="reversed"))),
                     np.max(np.abs(want - solve(tri_solver="thomas"))))
-        tol = max(1e-10, 10.0 * floor)
+        tol = max(1e-10, FLOOR_FACTOR * floor)
         got = hb.state_view(k).to_numpy().reshape(o.spots.size, o.vars.size)
         err = float(np.max(np.abs(got - want)))
         dp = abs(got[o.idx] - want[o.idx])
         print("option %d (%dx%d): grid err %.3e tol %.3e floor %.3e; price abs %.3e rel %.3e"
               % (k, o.spots.size, o.vars.size, err, tol, floor, dp, dp / abs(want[o.idx])))
+        record(err=err, floor=float(floor), tol=tol, ratio=err / floor, price_rel=dp / abs(want[o.idx]),
+               case="config 2 option %d (%dx%d)" % (k, o.spots.size, o.vars.size))
         assert err <= tol and dp <= tol

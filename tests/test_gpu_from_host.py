@@ -19,7 +19,7 @@ from oracle.blackscholes import BlackScholesOracle, BlackScholesParams
 from oracle.engine import HestonOracle, HestonParams
 from oracle.heston_cf import bs_call
 from fd_adi_amd.utils import sinh_space, exponential_space
-from tests.parity import check
+from tests.parity import check, PROBES
 
 pytestmark = pytest.mark.gpu
 
@@ -51,8 +51,9 @@ def test_black_scholes_1d(gpu, scheme):
     dt = 1.0 / 150
     V = run(G, scheme, 150, dt, U0)
     W = run(O, scheme, 150, dt, U0)
-    # rounding floor: the oracle with Thomas elimination and reversed sums
-    W_alt = run(bs_case(tri_solver="thomas", apply_order="reversed"), scheme, 150, dt, U0)
+    # rounding floor: the oracle with Thomas elimination, with reversed sums
+    W_alt = [run(bs_case(**kw), scheme, 150, dt, U0)
+             for kw in (dict(tri_solver="thomas"), dict(apply_order="reversed"))]
     print(scheme, end=": ")
     check(V, W, W_alt, O.idx)
     npt.assert_allclose(V[O.idx], bs_call(100.0, 99.0, 0.06, 0.2, 1.0), rtol=1e-3)
@@ -91,8 +92,11 @@ def test_heston_operators_from_host(gpu, scheme):
     U0 = O.payoff()
     V = run(G, scheme, 10, 0.05, U0)
     W = run(O, scheme, 10, 0.05, U0)
-    alt = heston_host()
-    alt.firsts_reversed = True
-    W_alt = run(alt, scheme, 10, 0.05, U0)
+    W_alt = []
+    for kw in PROBES:
+        alt = heston_host()
+        for k, v in kw.items():
+            setattr(alt, k, v)
+        W_alt.append(run(alt, scheme, 10, 0.05, U0))
     print(scheme, end=": ")
     check(V, W, W_alt, O.idx)

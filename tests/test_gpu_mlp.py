@@ -1,6 +1,7 @@
-"""Surrogate MLP on the GPU: MFMA GEMM vs NumPy, gradients vs the NumPy
-reference (bf16 rounding emulated), and training on ADI-generated labels
-beating the best constant and a linear model."""
+"""Surrogate MLP kernels on the GPU: the MFMA GEMM against NumPy and the
+gradients against the NumPy reference (bf16 rounding emulated), at the
+training batch of the surrogate among others.  Training on ADI labels,
+labels and serving are in tests/test_gpu_surrogate.py."""
 
 import numpy as np
 import numpy.testing as npt
@@ -35,9 +36,10 @@ def test_gemm_nt_matches_numpy(fd):
     npt.assert_allclose(got, want, rtol=1e-5, atol=1e-4)
 
 
-@pytest.mark.parametrize("B", [256, 2048])
+@pytest.mark.parametrize("B", [256, 2048, 8192])
 def test_gradients_match_reference(fd, B):
-    """B = 2048 exercises the split-K weight-gradient GEMMs (4 splits)."""
+    """B = 2048 exercises the split-K weight-gradient GEMMs (4 splits); 8192
+    is the training batch of the surrogate, 16 splits of 512 rows."""
     from fd_adi_amd.surrogate import MLP, init_params
     from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
     rng = np.random.default_rng(1)
@@ -56,86 +58,3 @@ def test_gradients_match_reference(fd, B):
     for a, b in blocks:
         rel = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
         assert rel < 2e-2, (a, b, rel)
-
-
-def test_training_on_adi_labels_beats_baselines(fd):
-    """The network trained on ADI labels must beat the best constant and the
-    best linear model on held-out queries (what the surrogate gains)."""
-    from fd_adi_amd.surrogate import MLP, adi_prices, within_call_bounds, cf_price, BOX_LO, BOX_HI
-    rng = np.random.default_rng(5)
-    n0, nv = 4096 + 1024 + 64, 16
-    P = rng.uniform(BOX_LO, BOX_HI, (n0, 8))
-    y = adi_prices(P, nS=48, nv=nv, nt=40)
-    # the coarse label mesh is unstable for some advection-dominated rows:
-    # keep the labels within the call bounds and within 0.5 of the exact
-    # price (the 48 x 16 mesh's discretisation error is ~2e-2)
-    ok = within_call_bounds(P, y) & (np.abs(y - cf_price(P)) < 0.5)
-    assert ok.mean() > 0.9, ok.mean()
-    P, y = P[ok][:4096 + 1024], y[ok][:4096 + 1024]
-    n = P.shape[0]
-    tr, te = slice(0, 4096), slice(4096, n)
-    mu, sd = y[tr].mean(), y[tr].std()
-    m = MLP(BOX_LO, BOX_HI, mu, sd, seed=0, max_rows=1024)
-    hist = m.fit(P[tr], y[tr], steps=400, batch=1024, lr=1e-3, log_every=100)
-    pred = m.predict(P[te])
-    err_mlp = np.sqrt(np.mean((pred - y[te]) ** 2))
-    err_const = np.sqrt(np.mean((mu - y[te]) ** 2))
-    X = np.hstack([P[tr], np.ones((4096, 1))])
-    coef = np.linalg.lstsq(X, y[tr], rcond=None)[0]
-    err_lin = np.sqrt(np.mean((np.hstack([P[te], np.ones((1024, 1))]).dot(coef) - y[te]) ** 2))
-    print("loss history", hist, "rmse mlp %.4f linear %.4f const %.4f" % (err_mlp, err_lin, err_const))
-    assert hist[-1][1] < hist[0][1]
-    assert err_mlp < 0.5 * err_lin < err_const
-
-
-def test_batched_labels_match_oracle(fd):
-    """The batched engine's labels on the surrogate's label meshes equal the
-    oracle's Craig-Sneyd solve on the same meshes (32 random rows), read at
-    (S0, v0) by the same interpolation, within the grid criterion of
-    tests/parity.py: the label
-    meshes are coarse and some parameter rows are nearly unstable, where the
-    oracle's own rounding noise (reordered summation) reaches 1e-6."""
-    from fd_adi_amd.surrogate import adi_prices, label_grids, label_price, BOX_LO, BOX_HI
-    from oracle.engine import HestonParams, HestonOracle
-    from tests.parity import grid_tolerance
-    rng = np.random.default_rng(11)
-    P = rng.uniform(BOX_LO, BOX_HI, (32, 8))
-    nS, nv, nt = 48, 16, 40
-    y = adi_prices(P, nS=nS, nv=nv, nt=nt)
-    S, V = label_grids(P, nS, nv)
-    rels = []
-    for k, r in enumerate(P):
-        hp = HestonParams(spot=100.0, strike=100 * r[7], r=r[5], v0=r[4], T=r[6], kappa=r[0],
-                          theta=r[1], sigma=r[2], rho=r[3])
-        E = HestonOracle(hp, S[k], V[k])
-        Ea = HestonOracle(hp, S[k], V[k], firsts_reversed=True)
-        W = E.solve_craigsneyd(nt, r[6] / nt, E.payoff(), 0.5)
-        Wa = Ea.solve_craigsneyd(nt, r[6] / nt, Ea.payoff(), 0.5)
-        tol, floor = grid_tolerance(W, Wa)
-        want = label_price(W[None], S[k][None], V[k][None], 100.0, r[4])[0]
-        assert abs(y[k] - want) <= tol, (k, y[k], want, tol, floor)
-        rels.append(abs(y[k] - want) / abs(want))
-    print("label rel errors: median %.2e max %.2e" % (np.median(rels), np.max(rels)))
-    assert np.median(rels) < 1e-13
-
-
-def test_serving_gate_and_fallback(fd):
-    """SurrogatePricer: queries the gate accepts get baseline + ensemble mean,
-    the others (and every query outside the training box) get the FP64 ADI
-    price on the label meshes, exactly."""
-    from fd_adi_amd import surrogate as S
-    P = S.draw_params(2048, 21)
-    y = S.adi_prices(P, nS=48, nv=16, nt=40)
-    ok = S.within_call_bounds(P, y) & (np.abs(y - S.cf_price(P)) < 0.5)
-    ens, _ = S.train_ensemble(P[ok], y[ok], members=2, steps=60, batch=1024, max_resid=0.5)
-    Q = S.draw_params(256, 22, outside_frac=0.25)
-    mean, spread = ens.predict(Q)
-    tau = float(np.median(spread))          # about half the queries pass the spread test
-    pricer = S.SurrogatePricer(ens, tau)
-    price, accepted = pricer.price(Q, nS=48, nv=16, nt=40)
-    inside = np.all((Q >= S.BOX_LO) & (Q <= S.BOX_HI), axis=1)
-    npt.assert_array_equal(accepted, inside & (spread <= tau))
-    assert 0 < accepted.sum() < inside.sum()
-    npt.assert_array_equal(price[accepted], mean[accepted])
-    truth = S.adi_prices(Q[~accepted], nS=48, nv=16, nt=40)
-    npt.assert_array_equal(price[~accepted], truth)

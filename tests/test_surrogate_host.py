@@ -44,3 +44,41 @@ def test_feller_ratio():
     P = np.array([[2.0, 0.04, 0.4, -0.5, 0.04, 0.03, 1.0, 1.0]])
     npt.assert_allclose(feller_ratio(P), [2 * 2.0 * 0.04 / 0.16])
     assert np.all(BOX_LO < BOX_HI)
+
+
+def test_calibrate_threshold_ties_and_empty():
+    from fd_adi_amd.surrogate import calibrate_threshold
+    # the two lowest scores are tied and one of them fails: nothing qualifies
+    tau, frac = calibrate_threshold([1.0, 1.0, 2.0], [0.0, 1.0, 0.0], atol=0.5)
+    assert tau == -np.inf and frac == 0.0
+    # the cut stops below the first failure
+    tau, frac = calibrate_threshold([4.0, 1.0, 3.0, 2.0], [0.0, 0.0, 1.0, 0.0], atol=0.5)
+    assert tau == 2.0 and frac == 0.5
+    # a tie straddling the cut moves it below the tied value
+    tau, frac = calibrate_threshold([1.0, 2.0, 2.0, 3.0], [0.0, 0.0, 1.0, 0.0], atol=0.5)
+    assert tau == 1.0 and frac == 0.25
+    # a failure budget of 20%: one failure in ten admitted
+    s = np.arange(1.0, 11.0)
+    e = np.zeros(10)
+    e[2] = 1.0
+    assert calibrate_threshold(s, e, atol=0.5, max_fail=0.2) == (10.0, 1.0)
+    assert calibrate_threshold([], [], atol=0.5) == (-np.inf, 0.0)
+
+
+def test_gate_rejects_everything_without_a_threshold():
+    from fd_adi_amd.surrogate import SurrogatePricer, Ensemble
+    P = draw_params(8, 1)
+
+    class One(object):
+        members = [None]
+
+        def predict(self, P):
+            return np.zeros(len(P)), np.zeros(len(P))
+
+    # a one-network ensemble has no spread: refused without an error model
+    import pytest
+    with pytest.raises(ValueError):
+        SurrogatePricer(One(), 0.0)
+    pr = SurrogatePricer(One(), -np.inf, error_model=lambda P: np.zeros(len(P)))
+    assert not pr.gate(P, np.zeros(len(P))).any()
+    assert isinstance(Ensemble([None, None]).members, list)
