@@ -49,6 +49,7 @@ struct HOpt {
   int64_t so, vo;       // offsets into the spot / variance tables
   double r, kappa, theta, sigma, rho, dt;
   double tdt;           // weight of the implicit part (theta*dt, or dt for implicit)
+  double th;            // the scheme's theta
   double fold_f;        // bottom fold factor of I - tdt A1 (lines with S != 0)
   double rdS0;          // 1 / (1 + tdt r/2): the S = 0 variance line
   double Gz;            // A1[nv-1, nv-3] for S != 0
@@ -289,10 +290,19 @@ struct Win {
   double v[3][3];
 };
 
+// A1 on one variance column scaled by a step weight c (c = dt for the
+// explicit operator dt A1, c = theta dt for theta dt A1): the entries the
+// reference's scaled operator holds, fl(c * a), for lines with S != 0 (sub,
+// mid, sup and, on the last column, z = the j-2 entry of the second-order
+// backward row) and the diagonal c * (-r/2) of the S = 0 line.
+struct ColOp {
+  double sub, mid, sup, z, s0;
+};
+
 struct VarCol {  // variance coefficients of column j (constant along a spot line)
-  double gsub, gmid, gsup, wv0, wv1, wv2, v;
+  double wv0, wv1, wv2, v;
   double rsv;     // rho * sigma * v
-  double lastGz;  // A1[nv-1, nv-3] on the last column, else 0
+  ColOp f1, t1;   // dt A1 and theta dt A1 on this column
 };
 
 // A0 bands of row i at variance v (Dirichlet handling is in the bands).
@@ -310,18 +320,27 @@ __device__ __forceinline__ Bands a0_bands(const SpotRow &R, double v) {
   return b;
 }
 
-// (A0 f)(i,j) from the window.
-__device__ __forceinline__ double a0w(const Bands &B, const Win &w) {
-  return B.sub * w.v[0][1] + B.mid * w.v[1][1] + B.sup * w.v[2][1];
+// (c A0 f)(i,j) from the window: the entries scaled first, fl(c * a), as the
+// reference's operator c A0 holds them, then summed in its band order (super,
+// main, sub).  The association matters where entries are large: the
+// reference scales the operator and applies it, so applying A0 and scaling
+// the result is a different rounding, which adds up step after step.
+__device__ __forceinline__ double a0s(double c, const Bands &B, const Win &w) {
+  const double sup = c * B.sup, mid = c * B.mid, sub = c * B.sub;
+  return (sup * w.v[2][1] + mid * w.v[1][1]) + sub * w.v[0][1];
 }
 
-// (A1 f)(i,j); fjm2 = f(i, j-2) (used on the last column only).
-__device__ __forceinline__ double a1w(const HOpt &o, const SpotRow &R, const VarCol &q,
-                                      const Win &w, double fjm2) {
-  // S = 0 lines keep only the -r/2 of the split
-  double acc = (R.snz != 0.0 ? q.gmid : -0.5 * o.r) * w.v[1][1];
-  acc += R.snz * (q.gsub * w.v[1][0] + q.gsup * w.v[1][2] + q.lastGz * fjm2);
-  return acc;
+__device__ __forceinline__ ColOp col_op(double c, double gsub, double gmid, double gsup,
+                                        double gz, double split) {
+  return ColOp{c * gsub, c * gmid, c * gsup, c * gz, c * split};
+}
+
+// (c A1 f)(i,j) with the scaled entries K of this column; fjm2 = f(i, j-2)
+// (K.z is zero except on the last column).  S = 0 lines keep only the -r/2
+// of the split.
+__device__ __forceinline__ double a1s(const ColOp &K, double snz, const Win &w, double fjm2) {
+  const double full = ((K.sup * w.v[1][2] + K.mid * w.v[1][1]) + K.sub * w.v[1][0]) + K.z * fjm2;
+  return snz != 0.0 ? full : K.s0 * w.v[1][1];
 }
 
 // (A01 f)(i,j) = rho sigma S v (ws x wv) . f, zero on boundary rows and columns
@@ -514,12 +533,16 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   auto clamp_row = [&](int i) { return (unsigned)(i < 0 ? 0 : (i >= nS ? nS - 1 : i)); };
 
   VarCol q;
-  q.gsub = VT(c, o, VT_GSUB, jc); q.gmid = VT(c, o, VT_GMID, jc); q.gsup = VT(c, o, VT_GSUP, jc);
   q.wv0 = VT(c, o, VT_WSUB, jc); q.wv1 = VT(c, o, VT_WMID, jc); q.wv2 = VT(c, o, VT_WSUP, jc);
   q.v = VT(c, o, VT_V, jc);
   q.rsv = (o.rho * o.sigma) * q.v;
   const bool last = jc == nv - 1;
-  q.lastGz = last ? o.Gz : 0.0;
+  {
+    const double gsub = VT(c, o, VT_GSUB, jc), gmid = VT(c, o, VT_GMID, jc),
+                 gsup = VT(c, o, VT_GSUP, jc), gz = last ? o.Gz : 0.0, split = -o.r / 2.0;
+    q.f1 = col_op(o.dt, gsub, gmid, gsup, gz, split);
+    q.t1 = col_op(o.tdt, gsub, gmid, gsup, gz, split);
+  }
   const unsigned ujc = (unsigned)jc;
   const unsigned jm = jc > 0 ? ujc - 1 : 0u, jp = jc < nv - 1 ? ujc + 1 : ujc;
   const unsigned jm2 = jc > 1 ? ujc - 2 : 0u;
@@ -593,29 +616,39 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     if (MODE == SM_IMPL) {
       rhs = wU.v[1][1] + dt * B.r0;
     } else {
-      const double a0u = a0w(B, wU);
-      const double a1u = a1w(o, R, q, wU, uj2);
-      const double r0 = B.r0;
-      const double fu = dt * (a0u + r0) + dt * a1u + dt * a01w(R, q, wU);
-      rhs = wU.v[1][1] + fu;
-      if (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) ex = tdt * a1u;
+      // Y0 = U + dt A0 U + dt R0 + dt A01 U + dt A1 U, in the reference's
+      // order of its explicit operators (spot, cross, variance) with each
+      // operator's entries scaled by dt
+      const double y0 = ((wU.v[1][1] + (a0s(dt, B, wU) + dt * B.r0)) + dt * a01w(R, q, wU)) +
+                        a1s(q.f1, R.snz, wU, uj2);
       if (MODE == SM_FIRST) {
-        rhs -= tdt * a0u;
+        rhs = y0 - a0s(tdt, B, wU);
+        ex = a1s(q.t1, R.snz, wU, uj2);
       } else {
-        // Operators are linear: A (Y2 - U) = A Y2 - A U.
-        const double a0y = a0w(B, wY);
-        const double a1y = a1w(o, R, q, wY, yj2);
-        const double c01 = a01w(R, q, wY) - a01w(R, q, wU);
+        // the second-stage corrections act on the difference of the states
+        // (FiniteDifferenceEngine.pyx:744-765, 818-835, 881-894)
+        Win wD;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int b = 0; b < 3; ++b) wD.v[r][b] = wY.v[r][b] - wU.v[r][b];  // Y2 - U
+        const double dj2 = yj2 - uj2;
         if (MODE == SM_HV) {
-          const double fd = dt * (a0y - a0u) + dt * (a1y - a1u) + dt * c01;
-          rhs += 0.5 * fd - tdt * a0y;
-          ex = tdt * a1y;
+          // V = Y0 - 1/2 F (U - Y2) over the explicit operators (no residual)
+          const double v1 = ((y0 + 0.5 * a0s(dt, B, wD)) + 0.5 * (dt * a01w(R, q, wD))) +
+                            0.5 * a1s(q.f1, R.snz, wD, dj2);
+          rhs = v1 - a0s(tdt, B, wY);
+          ex = a1s(q.t1, R.snz, wY, yj2);
         } else if (MODE == SM_CS) {
-          rhs += (0.5 * dt) * c01 - tdt * a0u;
-        } else {  // SM_MCS
-          const double th = tdt / dt;
-          const double fd = dt * ((a0y - a0u) + r0) + dt * (a1y - a1u) + dt * c01;
-          rhs += (th * dt) * c01 + (0.5 - th) * fd - tdt * a0u;
+          rhs = (y0 + (0.5 * dt) * a01w(R, q, wD)) - a0s(tdt, B, wU);
+          ex = a1s(q.t1, R.snz, wU, uj2);
+        } else {  // SM_MCS: (1/2 - theta) F (Y2 - U), residual included
+          const double h = 0.5 - o.th;
+          const double v1 = (((y0 + tdt * a01w(R, q, wD)) + h * (a0s(dt, B, wD) + dt * B.r0)) +
+                             h * (dt * a01w(R, q, wD))) +
+                            h * a1s(q.f1, R.snz, wD, dj2);
+          rhs = v1 - a0s(tdt, B, wU);
+          ex = a1s(q.t1, R.snz, wU, uj2);
         }
       }
     }
@@ -1396,6 +1429,7 @@ int refactor(fdadi_hb *hb, hipStream_t s) {
     HOpt &o = hb->h_opts[k];
     o.dt = hb->base_dt[k] * hb->dt_scale;
     o.tdt = hb->scheme == FDADI_SCHEME_IMPLICIT ? o.dt : hb->theta * o.dt;
+    o.th = hb->scheme == FDADI_SCHEME_IMPLICIT ? 1.0 : hb->theta;
   }
   FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
                                hipMemcpyHostToDevice, s));

@@ -7,7 +7,7 @@ Restates FiniteDifference/FiniteDifferenceEngine.pyx (operator construction
 dictionaries of FiniteDifference/heston.py:302-359, for the central scheme.
 Meshes are inputs: the caller supplies the spot and variance grids.
 
-Two reference quirks are exposed as switches (both default to the behaviour
+Reference quirks and rounding-order probes are exposed as switches (both default to the behaviour
 of the reference's GPU engine, which is what the product replaces):
 
 * ``r_split_float32`` -- the CPU engine adds -r/ndim through
@@ -240,6 +240,13 @@ class HestonOracle(object):
             for L in Les + Lis:
                 if L.is_foldable():
                     L.diagonalize()
+        elif self.tri_solver != "lapack":
+            # the elimination probes solve tridiagonal systems: fold the
+            # implicit operators (the solution is unchanged) so that the
+            # variance systems are eliminated without pivoting too
+            for L in Lis:
+                if L.is_foldable():
+                    L.diagonalize()
         return Les, Lis
 
     def cross_term(self, V):
@@ -256,6 +263,8 @@ class HestonOracle(object):
         Lis = [(self.operators[k] * -dt).add(1, inplace=True) for k in self._dims()]
         for L in Lis:
             L.solver = self.tri_solver
+            if self.tri_solver != "lapack" and L.is_foldable():
+                L.diagonalize()     # eliminated without pivoting (see _les_lis)
         if not self.implicit_residual:
             for L in Lis:
                 L.R = None
