@@ -1,22 +1,29 @@
 #!/usr/bin/env python
 """Throughput of the MI355X Heston ADI engine (one GPU).
 
-Default workload (the line's ``value``): a single FP64 Heston European call
-on a 2048 x 1024 (spot x variance) grid, rho = -0.7, Hundsdorfer-Verwer
-theta = 0.5, nt = 2000 steps (S0 = K = 100, r = 0.06, kappa = 1,
-theta = v0 = 0.04, sigma = 0.2, spot mesh on [0, 5K], variance mesh on
-[0, 5 theta]).  One bench "step" is one full option solve (2000 ADI steps) on
-device-resident inputs; ``value`` is options/s over the K timed solves.
+The metric is options/s at <= 1e-4 absolute error against FP64 ADI, for the
+surrogate with its ADI fallback, the batched ADI engine and the plain
+single-option engine.  ``value`` is the first of these at the surrogate
+serving configuration's full size: 10^6 fresh seed-drawn queries (5% of them
+outside the training box) answered by the surrogate trained on 2^20 FP64 ADI
+labels (256 x 128 grids, nt = 200, Craig-Sneyd), every query the gate
+rejects priced by the batched FP64 ADI engine on the same meshes.  One bench
+"step" is one pass of those 10^6 queries; ``value`` = queries / wall time
+(inputs already on the host, prices returned to the host).
 
 The line also carries
-* ``roofline``: the dominant kernel's algorithmic bytes per launch over its
-  HIP-event-measured average duration (events on the engine's stream, inside
-  the timed region), against the 8 TB/s HBM peak; ``traffic`` is read from
-  the rocprofv3 counter summary committed under profiles/ when present;
-* ``cpu_baseline``: the NumPy/SciPy restatement of the reference CPU engine
-  (oracle/), one thread, on a bounded sample of the same workload;
-* ``extra``: the 150 x 150 single option (plain loop) and the 4096-option
-  batched Craig-Sneyd workload in options/s.
+* ``roofline``: the dominant kernel of the served workload (a spot sweep of
+  the batched fallback engine), its algorithmic bytes per launch over its
+  mean duration from HIP events recorded on the engine's stream inside the
+  timed region, against the 8 TB/s HBM peak; ``traffic`` from the committed
+  rocprofv3 counter summary;
+* ``cpu_baseline``: the oracle (the NumPy/SciPy restatement of the reference
+  CPU engine) pricing options of the same workload, one option per process
+  over the host cores, on a bounded sample;
+* ``extra``: the 2048 x 1024 single option (roofline per ADI stage), the
+  150 x 150 plain option, the 4096-option batched workload and the
+  single-option loops, label throughput, training, the gate and the errors of
+  the admitted answers against FP64 ADI, the other CPU baselines.
 """
 
 import argparse
@@ -33,11 +40,14 @@ sys.path.insert(0, ROOT)
 METRIC = ("options/sec at ≤1e-4 abs err vs FP64 ADI: surrogate+fallback vs batched "
           "vs plain")
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per grid point for each HV stage (reads + writes of state
+# Algorithmic bytes per grid point of each stage (reads + writes of state
 # vectors; coefficients and factors are regenerated and count 0).
 HV_STAGE_BYTES = [16, 16, 24, 16]
 HV_STAGE_NAMES = ["spot_sweep_kernel<SM_FIRST>", "var_sweep_kernel<VM_PLAIN>",
                   "spot_sweep_kernel<SM_HV>", "var_sweep_kernel<VM_PLAIN>"]
+CS_STAGE_BYTES = [16, 16, 24, 16]
+CS_STAGE_NAMES = ["spot_sweep_kernel<SM_FIRST>", "var_sweep_kernel<VM_PLAIN>",
+                  "spot_sweep_kernel<SM_CS>", "var_sweep_kernel<VM_PLAIN>"]
 
 
 def log(*a):
@@ -56,11 +66,9 @@ def run_single(opt, scheme, theta, K, W, smoothing=0, graph=False):
     steps.  With ``graph`` the steps of a solve are one HIP graph replay
     (captured once, untimed); the last timed solve launches its stages one by
     one with a HIP event pair around each (the per-kernel durations)."""
-    import numpy as np
     import fd_adi_amd
     hb = fd_adi_amd.HestonBatch([opt.params], [opt.spots], [opt.vars], [opt.dt])
-    U0 = opt.payoff()
-    hb.set_state([U0])
+    hb.set_state([opt.payoff()])
     init = hb.state_view(0).copy(True)          # payoff kept in HBM
     view = hb.state_view(0)
     n = opt.nt - smoothing
@@ -96,11 +104,10 @@ def run_single(opt, scheme, theta, K, W, smoothing=0, graph=False):
     hb.sync()
     wall = time.perf_counter() - t0
     price = hb.get_state()[0][opt.idx]
-    return wall, stages, price, hb
+    return wall, stages, price
 
 
 def batched(n_opt, seed, K, W):
-    import numpy as np
     import fd_adi_amd
     from fd_adi_amd.workloads import batch_draws
     t = time.perf_counter()
@@ -111,7 +118,9 @@ def batched(n_opt, seed, K, W):
     hb.set_state([o.payoff() for o in opts])
     init = hb.state_view().copy(True)
     view = hb.state_view()
+    t = time.perf_counter()
     hb.configure("cs", 0.5, 1.0)
+    configure_s = time.perf_counter() - t
     shapes = hb.autotune()
     log("batch: autotuned launch shapes %s" % shapes)
     hb.step(2)
@@ -127,101 +136,7 @@ def batched(n_opt, seed, K, W):
     hb.sync()
     wall = time.perf_counter() - t0
     npts = sum(o.spots.size * o.vars.size for o in opts)
-    return n_opt * K / wall, wall / K, npts, shapes
-
-
-def surrogate_run(n_labels, n_queries, members, steps, seed=17, subsample=1.0):
-    """Configs 3-4 at reduced scale: FP64 ADI labels from the batched engine
-    (256 x 128, nt = 200, Craig-Sneyd), an ensemble of MLPs trained on the
-    GPU, the spread gate calibrated on held-out labels for 1e-4 absolute
-    error, then serving of fresh queries (5% outside the training box) with
-    the ADI fallback; errors against FP64 ADI on a sample of the queries."""
-    import numpy as np
-    import fd_adi_amd
-    from fd_adi_amd import surrogate as S
-    out = {"labels": n_labels, "queries": n_queries, "members": members,
-           "train_steps": steps, "batch": 8192, "grid": "256x128 nt=200 cs, spot mesh K x fixed sinh mesh, fixed "
-                                     "variance power mesh, bicubic read-out at (S0, v0)"}
-    t = time.perf_counter()
-    P = S.draw_params(n_labels, seed)
-    y = S.adi_prices(P)
-    fd_adi_amd._lib.sync()
-    t_lab = time.perf_counter() - t
-    ok = S.within_call_bounds(P, y)
-    out["label_options_per_s"] = round(n_labels / t_lab, 1)
-    out["labels_dropped_unstable"] = int((~ok).sum())
-    P, y = P[ok], y[ok]
-    nval = max(len(y) // 10, 1)
-    Pv, yv, Pt, yt = P[:nval], y[:nval], P[nval:], y[nval:]
-    t = time.perf_counter()
-    feller_min, max_resid = 0.0, 0.05
-    out["train_subsample"] = subsample
-    out["gate_feller_min"], out["train_max_resid"] = feller_min, max_resid
-    ens, hists = S.train_ensemble(Pt, yt, members=members, steps=steps, log_every=steps // 4,
-                                  max_resid=max_resid, schedule="cosine", subsample=subsample)
-    fd_adi_amd._lib.sync()
-    t_train = time.perf_counter() - t
-    flop = 4.75e6 * 8192 * steps * members
-    out["train_sec"] = round(t_train, 2)
-    out["train_TFLOPs"] = round(flop / t_train / 1e12, 1)
-    out["train_final_loss"] = [round(h[-1][1], 6) for h in hists]
-    out["baseline"] = "semi-analytic Heston price (device), network learns ADI - it"
-    base_v = S.cf_price(Pv)
-    out["baseline_only_val_rmse"] = float(np.sqrt(np.mean((base_v - yv) ** 2)))
-    out["baseline_only_val_max_abs_err"] = float(np.max(np.abs(base_v - yv)))
-    mean_v, spread_v = ens.predict(Pv)
-    out["val_rmse"] = float(np.sqrt(np.mean((mean_v - yv) ** 2)))
-    out["val_max_abs_err"] = float(np.max(np.abs(mean_v - yv)))
-    # ungated: how many held-out labels the ensemble mean (and the analytic
-    # price alone) meets at each tolerance
-    out["val_frac_within"] = {"%g" % a: round(float(np.mean(np.abs(mean_v - yv) <= a)), 4)
-                              for a in (1e-4, 1e-3, 1e-2)}
-    out["baseline_only_val_frac_within"] = {
-        "%g" % a: round(float(np.mean(np.abs(base_v - yv) <= a)), 4) for a in (1e-4, 1e-3, 1e-2)}
-    tau, acc = S.calibrate_gate(ens, Pv, yv, 1e-4, feller_min)
-    out["gate_tau"] = tau
-    out["gate_val_accept"] = round(acc, 4)
-    # the same gate calibrated for looser tolerances (not used for serving)
-    out["val_accept_at_atol"] = {"%g" % a: round(S.calibrate_gate(ens, Pv, yv, a,
-                                                                   feller_min)[1], 4)
-                                 for a in (1e-3, 1e-2, 1e-1)}
-    out["val_accept_at_atol_feller_ge_1.5"] = {
-        "%g" % a: round(S.calibrate_gate(ens, Pv, yv, a, 1.5)[1], 4) for a in (1e-4, 1e-3, 1e-2)}
-    # a looser gate (not used for serving): the largest spread threshold at
-    # which at most 0.1% of the accepted held-out queries exceed 1e-4
-    order = np.argsort(spread_v)
-    fails = np.cumsum(np.abs(mean_v - yv)[order] > 1e-4)
-    okq = np.where(fails <= 1e-3 * np.arange(1, len(order) + 1))[0]
-    out["val_accept_1e-4_at_0.1pct_failures"] = round((okq[-1] + 1) / len(order), 4) \
-        if okq.size else 0.0
-    reg = S.feller_ratio(Pv) >= feller_min
-    out["val_rmse_in_gate_region"] = float(np.sqrt(np.mean((mean_v[reg] - yv[reg]) ** 2)))
-    out["baseline_only_val_rmse_in_gate_region"] = float(
-        np.sqrt(np.mean((base_v[reg] - yv[reg]) ** 2)))
-    pricer = S.SurrogatePricer(ens, tau, feller_min=feller_min)
-    Q = S.draw_params(n_queries, seed + 1, outside_frac=0.05)
-    t = time.perf_counter()
-    price, okq = pricer.price(Q)
-    fd_adi_amd._lib.sync()
-    t_serve = time.perf_counter() - t
-    out["options_per_s"] = round(n_queries / t_serve, 1)
-    out["fallback_rate"] = round(1.0 - okq.mean(), 4)
-    t = time.perf_counter()
-    _ = ens.predict(Q)
-    fd_adi_amd._lib.sync()
-    out["surrogate_only_options_per_s"] = round(n_queries / (time.perf_counter() - t), 1)
-    t = time.perf_counter()
-    _ = S.cf_price(Q)
-    fd_adi_amd._lib.sync()
-    out["semi_analytic_options_per_s"] = round(n_queries / (time.perf_counter() - t), 1)
-    rng = np.random.default_rng(seed + 2)
-    idx = rng.choice(n_queries, min(1024, n_queries), replace=False)
-    truth = S.adi_prices(Q[idx])
-    err = np.abs(price[idx] - truth)
-    out["err_sample"] = len(idx)
-    out["max_abs_err"] = float(np.max(err))
-    out["mean_abs_err"] = float(np.mean(err))
-    return out
+    return n_opt * K / wall, wall / K, npts, shapes, configure_s
 
 
 def single_loops(n_op, n_fused, seed=2024):
@@ -251,35 +166,102 @@ def single_loops(n_op, n_fused, seed=2024):
     return out
 
 
-def cpu_baseline(steps):
-    """Oracle (NumPy/SciPy restatement of the reference CPU engine), one
-    thread, ``steps`` HV steps of the 2048x1024 option; extrapolated to the
-    full 2000-step solve."""
-    code = r"""
-import os, sys, time, json
-sys.path.insert(0, %r)
-from fd_adi_amd.workloads import single
-from oracle.engine import HestonOracle, HestonParams
-o = single(2048, 1024, 2000, -0.7)
-E = HestonOracle(HestonParams(spot=o.S0, strike=o.K, r=o.r, v0=o.v0, T=o.T, kappa=o.kappa,
-                 theta=o.theta, sigma=o.sigma, rho=o.rho), o.spots, o.vars)
-U0 = o.payoff()
-E.solve_hundsdorferverwer(1, o.dt, U0)
-t = time.perf_counter()
-E.solve_hundsdorferverwer(%d, o.dt, U0)
-print(json.dumps({"sec_per_step": (time.perf_counter() - t) / %d}))
-""" % (ROOT, steps, steps)
+def surrogate_run(n_labels, n_queries, steps, members, max_fail, check, seed=17,
+                  after_labels=None):
+    """Configurations 3 and 4 at their sizes: FP64 ADI labels from the
+    device labeller, the gated surrogate (ensemble, error model, calibrated
+    threshold), then serving of fresh queries with the ADI fallback (the
+    timed step); the admitted answers are checked against FP64 ADI on a
+    sample of ``check`` queries (all of them with check < 0)."""
+    import numpy as np
+    import fd_adi_amd
+    from fd_adi_amd import surrogate as S
+    out = {"labels": n_labels, "queries": n_queries, "members": members, "train_steps": steps,
+           "batch": 8192, "atol": 1e-4, "gate_max_fail": max_fail,
+           "grid": "256x128, nt=200, Craig-Sneyd; spot mesh K x one unit sinh mesh, one "
+                   "power-law variance mesh; bicubic read-out at (S0, v0)"}
+    lab = S.labeler().warm()                      # engines, launch shapes, graphs: untimed
+    P = S.draw_params(n_labels, seed)
+    lab.price(P[:4096])
+    fd_adi_amd._lib.sync()
+    t = time.perf_counter()
+    y = lab.price(P)
+    fd_adi_amd._lib.sync()
+    t_lab = time.perf_counter() - t
+    out["label_sec"] = round(t_lab, 2)
+    out["label_options_per_s"] = round(n_labels / t_lab, 1)
+    out["label_GBs_algorithmic"] = round(72 * 256 * 128 * 200 * n_labels / t_lab / 1e9, 1)
+    log("surrogate: %d labels in %.1fs" % (n_labels, t_lab))
+    if after_labels is not None:
+        out["cpu_baselines"] = after_labels()
+    t = time.perf_counter()
+    pricer, info = S.fit_surrogate(P, y, members=members, steps=steps, max_fail=max_fail,
+                                   seed=seed)
+    fd_adi_amd._lib.sync()
+    t_fit = time.perf_counter() - t
+    out["fit_sec"] = round(t_fit, 2)
+    out["train_TFLOPs"] = round(4.75e6 * 8192 * steps * members / t_fit / 1e12, 1)
+    out.update(info)
+    log("surrogate: fit %.1fs %s" % (t_fit, info))
+    Q = S.draw_params(n_queries, seed + 1, outside_frac=0.05)
+    pricer.price(Q[:4096])                        # warm-up of the serving path
+    fd_adi_amd._lib.sync()
+    t = time.perf_counter()
+    price, ok = pricer.price(Q, timed=True)
+    fd_adi_amd._lib.sync()
+    t_serve = time.perf_counter() - t
+    out["serve_sec"] = round(t_serve, 2)
+    out["options_per_s"] = round(n_queries / t_serve, 1)
+    out["fallback_rate"] = round(1.0 - float(ok.mean()), 4)
+    out["fallback_stage_ms"] = [round(x, 4) for x in lab.last_stage_ms]
+    out["fallback_stage_batch"] = lab.last_batch
+    t = time.perf_counter()
+    mean, spread = pricer.ensemble.predict(Q)
+    fd_adi_amd._lib.sync()
+    out["surrogate_only_options_per_s"] = round(n_queries / (time.perf_counter() - t), 1)
+    # the admitted answers against FP64 ADI (the fallback answers are FP64 ADI)
+    acc = np.nonzero(ok)[0]
+    if check >= 0 and acc.size > check:
+        acc = np.random.default_rng(seed + 2).choice(acc, check, replace=False)
+    if acc.size:
+        truth = lab.price(Q[acc])
+        err = np.abs(price[acc] - truth)
+        out["checked_admitted"] = int(acc.size)
+        out["admitted_max_abs_err"] = float(err.max())
+        out["admitted_mean_abs_err"] = float(err.mean())
+        out["admitted_frac_within_atol"] = float(np.mean(err <= 1e-4))
+    else:
+        out["checked_admitted"] = 0
+    return out
+
+
+def cpu_baselines(procs, n_label, n_batch, c1_steps):
+    """The oracle timed on the host cores, in child processes (started early,
+    run while the GPU works): config 0 and config 1 on one core; the
+    surrogate's fallback options (label meshes) and the batched workload's
+    options one per process over ``procs`` processes."""
     env = dict(os.environ)
     for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS",
               "NUMEXPR_NUM_THREADS", "BLIS_NUM_THREADS"):
         env[v] = "1"
     env["HIP_VISIBLE_DEVICES"] = ""
-    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
-                         timeout=600)
-    if out.returncode != 0:
-        raise RuntimeError(out.stderr[-2000:])
-    sec = json.loads(out.stdout.strip().splitlines()[-1])["sec_per_step"]
-    return sec
+    script = os.path.join(ROOT, "oracle", "cpu_baseline.py")
+    cmds = {"config0": ["single", "config0"], "config1": ["single", "config1", str(c1_steps)],
+            "labels": ["labels", str(n_label), str(procs)],
+            "batch": ["batch", str(n_batch), str(procs)]}
+    return {k: subprocess.Popen([sys.executable, script] + a, env=env, stdout=subprocess.PIPE,
+                                stderr=subprocess.PIPE, text=True)
+            for k, a in cmds.items()}
+
+
+def collect(procs):
+    out = {}
+    for k, p in procs.items():
+        so, se = p.communicate(timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError("cpu baseline %s failed: %s" % (k, se[-2000:]))
+        out[k] = json.loads(so.strip().splitlines()[-1])
+    return out
 
 
 def probe():
@@ -299,6 +281,25 @@ def probe():
     log("probe: 20 HV steps (80 sweep launches) + 5 scale launches on 2^25 doubles done")
 
 
+def probe_labels():
+    """Fixed fallback-engine dispatches for counter runs (see --probe-labels)."""
+    import fd_adi_amd
+    from fd_adi_amd import surrogate as S
+    from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
+    lab = S.AdiLabeler(tune=True)
+    P = S.draw_params(4096, 5)
+    lab.price(P)
+    hb, _ = lab._engine(4096)
+    hb.step(20)
+    hb.sync()
+    x = SizedArrayPtr().alloc(1 << 25)
+    for _ in range(5):
+        x.timeseq_scalar_from_host(1.0)
+    fd_adi_amd._lib.sync()
+    log("probe-labels: one 4096-option batch (graph), 20 CS steps (80 sweep launches), "
+        "5 scale launches on 2^25 doubles done; shapes %s %s" % (hb.get_shape(0), hb.get_shape(1)))
+
+
 def probe_mlp():
     """Fixed MLP dispatch sequence for kernel traces (see --probe-mlp)."""
     import numpy as np
@@ -315,11 +316,11 @@ def probe_mlp():
 
 def probe_cf():
     """Fixed semi-analytic pricer dispatches for kernel traces (see --probe-cf)."""
+    import numpy as np
     import fd_adi_amd
     from fd_adi_amd import surrogate as S
     from fd_adi_amd.pricing import heston_call_device
     from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
-    import numpy as np
     P = S.draw_params(1 << 20, 3)
     rows = np.ascontiguousarray(np.stack([np.full(len(P), 100.0), 100.0 * P[:, 7], P[:, 5],
                                           P[:, 4], P[:, 6], P[:, 0], P[:, 1], P[:, 2],
@@ -331,9 +332,9 @@ def probe_cf():
     log("probe-cf: 5 launches pricing 2^20 options done")
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 summary."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_config1.txt")
+def read_traffic(profile, kernel):
+    """HBM bytes per launch of ``kernel`` from a committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -346,22 +347,31 @@ def read_traffic(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5, help="timed option solves")
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample-steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=1,
+                    help="timed passes over the serving queries")
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed passes over the serving queries before the timed ones")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="processes of the per-option CPU baselines (0: host cores - 2, <= 14)")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--surrogate-labels", type=int, default=1 << 18,
-                    help="ADI-labelled training draws for the surrogate (0 = skip)")
-    ap.add_argument("--surrogate-queries", type=int, default=1 << 15)
-    ap.add_argument("--surrogate-steps", type=int, default=8000)
-    ap.add_argument("--surrogate-subsample", type=float, default=1.0,
-                    help="fraction of the labels each ensemble member trains on (bagging)")
+    ap.add_argument("--config1-steps", type=int, default=5, help="timed 2048x1024 solves")
+    ap.add_argument("--surrogate-labels", type=int, default=1 << 20)
+    ap.add_argument("--surrogate-queries", type=int, default=1000000)
+    ap.add_argument("--surrogate-steps", type=int, default=16000)
+    ap.add_argument("--surrogate-members", type=int, default=4)
+    ap.add_argument("--gate-max-fail", type=float, default=0.0,
+                    help="failure budget of the gate's calibration (0: no calibration query "
+                         "beyond 1e-4 is admitted)")
+    ap.add_argument("--check-admitted", type=int, default=32768,
+                    help="admitted answers checked against FP64 ADI (-1: all of them)")
     ap.add_argument("--probe", action="store_true",
-                    help="profiling probe for rocprofv3 counter runs: 20 HV steps of the "
-                         "2048x1024 option, then 5 launches of the scale kernel on 2^25 "
-                         "doubles (reads and writes 256 MiB each: the copy reference); "
+                    help="profiling probe: 20 HV steps of the 2048x1024 option, then 5 "
+                         "launches of the scale kernel on 2^25 doubles (the copy reference); "
                          "prints no JSON line")
+    ap.add_argument("--probe-labels", action="store_true",
+                    help="profiling probe: one 4096-option batch of the fallback engine and "
+                         "20 more CS steps of it, then the scale kernel; no JSON line")
     ap.add_argument("--probe-mlp", action="store_true",
                     help="profiling probe: 50 Adam steps of one surrogate network at batch "
                          "8192 on random features and targets; prints no JSON line")
@@ -377,79 +387,130 @@ def main():
     import numpy as np
     import fd_adi_amd  # noqa: F401
 
-    if args.probe:
-        probe()
-        return
-    if args.probe_mlp:
-        probe_mlp()
-        return
-    if args.probe_cf:
-        probe_cf()
-        return
-    opt = config1()
-    N = opt.spots.size * opt.vars.size
-    log("config1: %dx%d, nt=%d, HV; warmup %d, timed %d" % (opt.spots.size, opt.vars.size,
-                                                           opt.nt, args.warmup, args.steps))
-    wall, stages, price, _ = run_single(opt, "hv", 0.5, args.steps, args.warmup, graph=True)
-    ms_per = 1e3 * wall / args.steps
-    value = args.steps / wall
-    per_launch_ms = stages / opt.nt
-    stage_gbs = [HV_STAGE_BYTES[k] * N / (per_launch_ms[k] * 1e-3) / 1e9 for k in range(4)]
-    dom = int(np.argmax(stages))
-    achieved = stage_gbs[dom]
-    log("config1: %.2f ms/option, price %.12f; stage ms/launch %s; GB/s %s" % (
-        ms_per, price, np.round(per_launch_ms, 4).tolist(), np.round(stage_gbs, 1).tolist()))
-    algo_step = sum(HV_STAGE_BYTES) * N
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic(HV_STAGE_NAMES[dom]),
-                "kernel": HV_STAGE_NAMES[dom], "bytes_per_launch": HV_STAGE_BYTES[dom] * N,
-                "avg_launch_us": round(per_launch_ms[dom] * 1e3, 3),
-                "stage_GBs": [round(x, 1) for x in stage_gbs],
-                "step_GBs": round(algo_step / (wall / args.steps / opt.nt) / 1e9, 1)}
-
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline: %d HV steps of the same option on one core" % args.cpu_sample_steps)
-        sec = cpu_baseline(args.cpu_sample_steps)
-        cpu = {"value": 1.0 / (sec * opt.nt), "unit": "options/s", "cores": 1, "kind": "port",
-               "sample": "%d of 2000 HV steps of the 2048x1024 option (oracle, NumPy/SciPy "
-                         "dia.dot + LAPACK solve_banded), extrapolated" % args.cpu_sample_steps,
-               "sec_per_step": sec}
+    for flag, fn in ((args.probe, probe), (args.probe_labels, probe_labels),
+                     (args.probe_mlp, probe_mlp), (args.probe_cf, probe_cf)):
+        if flag:
+            fn()
+            return
 
     extra = {}
+    # the 2048 x 1024 single option: per-stage roofline of the fused sweeps
+    opt = config1()
+    N = opt.spots.size * opt.vars.size
+    wall, stages, price = run_single(opt, "hv", 0.5, args.config1_steps, 1, graph=True)
+    per_launch_ms = stages / opt.nt
+    stage_gbs = [HV_STAGE_BYTES[k] * N / (per_launch_ms[k] * 1e-3) / 1e9 for k in range(4)]
+    payoff_max = float(opt.spots[-1] - opt.K)
+    extra["config1_2048x1024_hv"] = {
+        "options_per_s": round(args.config1_steps / wall, 4),
+        "ms_per_option": round(1e3 * wall / args.config1_steps, 3),
+        "stage_us": [round(1e3 * x, 2) for x in per_launch_ms],
+        "stage_GBs_algorithmic": [round(x, 1) for x in stage_gbs],
+        "step_GBs_algorithmic": round(sum(HV_STAGE_BYTES) * N /
+                                      (wall / args.config1_steps / opt.nt) / 1e9, 1),
+        "price": price, "price_valid": bool(np.isfinite(price) and 0 <= price <= payoff_max),
+        "note": "the reference algorithm diverges on this grid (tests/test_oracle_stability.py);"
+                " a throughput measurement, not a price"}
+    log("config1: %s" % extra["config1_2048x1024_hv"])
+
     if not args.no_extra:
         from fd_adi_amd.workloads import single
         o0 = single(150, 150, 150, 0.0, spot_max=500.0, var_max=0.2)
-        w0, _, p0, _ = run_single(o0, "douglas", 0.5, 20, 2, smoothing=2)
-        extra["plain_150x150_douglas_smoothed_options_per_s"] = round(20 / w0, 2)
-        extra["plain_150x150_price"] = p0
+        w0, _, p0 = run_single(o0, "douglas", 0.5, 20, 2, smoothing=2)
+        extra["plain_150x150_douglas_smoothed"] = {
+            "options_per_s": round(20 / w0, 2), "price": p0,
+            "launches_per_option": 2 * 4 + 2 * 148}
         loops = single_loops(2, 16)
         extra["single_option_loop_cs_options_per_s"] = loops
-        rate, sec, npts, shapes = batched(4096, 2024, 2, 1)
-        extra["batched_4096_shapes"] = {"spot_threads_cols": list(shapes[0]),
-                                        "var_lanes": shapes[1][0]}
-        extra["batched_4096_cs_options_per_s"] = round(rate, 1)
-        extra["batched_4096_cs_sec"] = round(sec, 4)
-        extra["batched_4096_cs_GBs_algorithmic"] = round(72 * npts * 200 / sec / 1e9, 1)
-        extra["batched_over_op_path_loop"] = round(rate / loops["op_path"], 1)
-        extra["batched_over_fused_loop"] = round(rate / loops["fused"], 1)
-        log("extra: %s" % extra)
+        rate, sec, npts, shapes, cfg_s = batched(4096, 2024, 3, 1)
+        extra["batched_4096_cs"] = {
+            "options_per_s": round(rate, 1), "sec_per_batch": round(sec, 4),
+            "configure_sec": round(cfg_s, 4),
+            "options_per_s_with_configure": round(4096 / (sec + cfg_s), 1),
+            "GBs_algorithmic": round(72 * npts * 200 / sec / 1e9, 1),
+            "shapes": {"spot_threads_cols": list(shapes[0]), "var_lanes": shapes[1][0]},
+            "over_op_path_loop": round(rate / loops["op_path"], 1),
+            "over_fused_loop": round(rate / loops["fused"], 1)}
+        log("extra: %s" % {k: extra[k] for k in extra if k != "config1_2048x1024_hv"})
+
+    # The CPU baselines run in child processes while the labels are generated
+    # (GPU-bound; the host waits): they would slow the host-bound
+    # measurements above, and are collected before anything timed on the
+    # host runs again.
+    cpu_procs = None
+    if not args.no_cpu:
+        nproc = args.cpu_procs or max(1, min(12, (os.cpu_count() or 4) - 4))
+        cpu_procs = cpu_baselines(nproc, 2 * nproc, 2 * nproc, 30)
+        log("cpu baselines started (%d processes for the per-option ones)" % nproc)
+
+    sur = None
     if args.surrogate_labels:
-        sur = surrogate_run(args.surrogate_labels, args.surrogate_queries, 4,
-                            args.surrogate_steps, subsample=args.surrogate_subsample)
+        sur = surrogate_run(args.surrogate_labels, args.surrogate_queries, args.surrogate_steps,
+                            args.surrogate_members, args.gate_max_fail, args.check_admitted,
+                            after_labels=lambda: collect(cpu_procs) if cpu_procs else None)
         extra["surrogate"] = sur
         log("surrogate: %s" % sur)
+        if "batched_4096_cs" in extra:
+            sur["over_batched_4096"] = round(sur["options_per_s"] /
+                                             extra["batched_4096_cs"]["options_per_s"], 2)
+        sur["over_fallback_engine"] = round(sur["options_per_s"] / sur["label_options_per_s"],
+                                            2)
+
+    cpu = None
+    if cpu_procs is not None:
+        base = sur.pop("cpu_baselines") if sur is not None else collect(cpu_procs)
+        extra["cpu_baselines"] = base
+        lab = base["labels"]
+        cpu = {"value": lab["options_per_s"], "unit": "options/s", "cores": lab["procs"],
+               "kind": "port",
+               "sample": "%d options on the serving workload's 256x128 meshes (nt=200, CS), one "
+                         "per process, oracle (NumPy/SciPy dia.dot + LAPACK solve_banded), "
+                         "host CPU %s" % (lab["options"], lab["cpu_model"])}
+
+    if sur is not None:
+        value = sur["options_per_s"]
+        ms_per = 1e3 * sur["serve_sec"]
+        nq = sur["queries"]
+        st = np.array(sur["fallback_stage_ms"]) / 200.0      # ms per launch
+        pts = sur["fallback_stage_batch"] * 256 * 128
+        gbs = [CS_STAGE_BYTES[k] * pts / (st[k] * 1e-3) / 1e9 for k in range(4)]
+        dom = int(np.argmax(st))
+        roofline = {"bound": "hbm", "achieved": round(gbs[dom], 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(gbs[dom] / HBM_PEAK_GBS, 4),
+                    "traffic": read_traffic("hbm_traffic_labels_4096x256x128.txt",
+                                            CS_STAGE_NAMES[dom]),
+                    "kernel": CS_STAGE_NAMES[dom],
+                    "bytes_per_launch": CS_STAGE_BYTES[dom] * pts,
+                    "avg_launch_us": round(st[dom] * 1e3, 2),
+                    "stage_us": [round(x * 1e3, 2) for x in st],
+                    "stage_GBs": [round(x, 1) for x in gbs]}
+        workload = "heston_surrogate_serving_%dq_%dlabels_256x128_cs_nt200" % (
+            nq, sur["labels"])
+        cfg = {"workload": workload, "queries": nq, "outside_box_frac": 0.05,
+               "labels": sur["labels"], "members": sur["members"],
+               "gate": "error model, calibrated at %g failures above 1e-4" % sur["gate_max_fail"],
+               "fallback": "batched FP64 ADI, 256x128, nt=200, Craig-Sneyd"}
+        steps, warmup = 1, 0
+    else:
+        value = extra["config1_2048x1024_hv"]["options_per_s"]
+        ms_per = extra["config1_2048x1024_hv"]["ms_per_option"]
+        dom = int(np.argmax(stages))
+        roofline = {"bound": "hbm", "achieved": round(stage_gbs[dom], 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
+                    "traffic": read_traffic("hbm_traffic_config1.txt", HV_STAGE_NAMES[dom]),
+                    "kernel": HV_STAGE_NAMES[dom],
+                    "bytes_per_launch": HV_STAGE_BYTES[dom] * N,
+                    "avg_launch_us": round(per_launch_ms[dom] * 1e3, 3)}
+        cfg = {"workload": "heston_call_2048x1024_hv_nt2000_rho-0.7", "nS": 2048, "nv": 1024,
+               "nt": 2000, "scheme": "hundsdorferverwer", "theta": 0.5}
+        steps, warmup = args.config1_steps, 1
 
     line = {
-        "metric": METRIC, "value": round(value, 4), "unit": "options/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per, 3),
+        "metric": METRIC, "value": round(value, 2), "unit": "options/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms_per, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "heston_call_2048x1024_hv_nt2000_rho-0.7", "nS": 2048,
-                   "nv": 1024, "nt": 2000, "scheme": "hundsdorferverwer", "theta": 0.5,
-                   "price": price},
-        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+        "data": "synthetic", "config": cfg, "roofline": roofline, "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
 

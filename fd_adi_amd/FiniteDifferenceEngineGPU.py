@@ -252,11 +252,13 @@ class FiniteDifferenceEngineADI(FiniteDifferenceEngine):
         return False
 
     def solve_implicit_(self, n, dt, V, callback=None, numpy=False):
-        """Fully implicit: cross term explicit, then variance and spot solves."""
+        """Fully implicit: cross term explicit, then variance and spot solves.
+        ``dt`` may be a device scalar (SizedArrayPtr), as in the reference
+        (FiniteDifferenceEngineGPU.pyx:379)."""
         if callback or numpy:
             raise NotImplementedError("Callbacks and Numpy not available for GPU solver.")
         if self._fused_ok():
-            return self._fused(V, "implicit", n, _host_scalar(dt), 1.0)
+            return self._fused(V, "implicit", n, dt, 1.0)
         f, l, Lis = self.preprocess_operators(dt, 1.0)
         del f, l
         for L in Lis:
@@ -311,9 +313,10 @@ class FiniteDifferenceEngineADI(FiniteDifferenceEngine):
         _lib.sync()
 
     def solve_hundsdorferverwer_(self, n, dt, V, theta=0.5):
-        theta_h = _host_scalar(theta)
+        """``dt`` and ``theta`` may be device scalars (SizedArrayPtr), as in
+        the reference (FiniteDifferenceEngineGPU.pyx:502)."""
         if self._fused_ok():
-            return self._fused(V, "hv", n, _host_scalar(dt), theta_h)
+            return self._fused(V, "hv", n, dt, theta)
         Firsts, Les, Lis = self.preprocess_operators(dt, theta)
         for L in itertools.chain(Les, Lis, Firsts):
             if L.is_foldable():
@@ -404,10 +407,16 @@ class FiniteDifferenceEngineADI(FiniteDifferenceEngine):
 
     def solve_smooth_(self, n, dt, V, smoothing_steps=2):
         """2*s implicit half steps, then HV at theta = 0.6 to completion
-        (FiniteDifferenceEngineGPU.pyx:582-593)."""
-        dt_h = _host_scalar(dt)
-        self.solve_implicit_(smoothing_steps * 2, 0.5 * dt_h, V)
-        self.solve_hundsdorferverwer_(int(n) - smoothing_steps, dt_h, V, 0.6)
+        (FiniteDifferenceEngineGPU.pyx:582-593); a device ``dt`` is halved on
+        the device, as the reference does."""
+        if isinstance(dt, SizedArrayPtr):
+            half = SizedArrayPtr().alloc(1)
+            half.copy_from(dt)
+            half.timeseq_scalar_from_host(0.5)
+        else:
+            half = 0.5 * float(dt)
+        self.solve_implicit_(smoothing_steps * 2, half, V)
+        self.solve_hundsdorferverwer_(int(n) - smoothing_steps, dt, V, 0.6)
 
     def _dummy(self):
         """One Douglas step from arange(N) (FiniteDifferenceEngineGPU.pyx:283-340)."""
@@ -547,10 +556,18 @@ class HestonFiniteDifferenceEngine(FiniteDifferenceEngineADI):
         return self._batch
 
     def _fused(self, V, scheme, n, dt, theta):
+        """Run on the one-option fused engine (unit base step, so dt is the
+        multiplier); with a device dt or theta the scheme is configured from
+        device memory and nothing is read back."""
         hb = self.batch()
         view = hb.state_view(0)
         view.copy_from(V)
-        hb.configure(scheme, theta, dt)
+        if isinstance(dt, SizedArrayPtr) or isinstance(theta, SizedArrayPtr):
+            dev = [x if isinstance(x, SizedArrayPtr) else SizedArrayPtr(np.atleast_1d(float(x)))
+                   for x in (theta, dt)]
+            hb.configure_dev(scheme, dev[0], dev[1])
+        else:
+            hb.configure(scheme, float(theta), float(dt))
         hb.step(int(n))
         V.copy_from(view)
         _lib.sync()

@@ -96,6 +96,7 @@ _SIGS = {
     "fdadi_hb_create": (c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, _P(c_vp)]),
     "fdadi_hb_destroy": (c_int, [c_vp]),
     "fdadi_hb_configure": (c_int, [c_vp, c_int, c_double, c_double, c_vp]),
+    "fdadi_hb_configure_dev": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
     "fdadi_hb_update": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
     "fdadi_hb_call_payoff": (c_int, [c_vp, c_vp, c_vp]),
     "fdadi_hb_read_bicubic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
@@ -110,6 +111,8 @@ _SIGS = {
     "fdadi_hb_launches_per_step": (c_int, [c_vp, _P(c_int)]),
     "fdadi_mlp_nparam": (c_int, [_P(c_i64)]),
     "fdadi_mlp_create": (c_int, [c_i32, c_vp, c_i64, c_vp, c_vp, c_double, c_double, _P(c_vp)]),
+    "fdadi_mlp_create_features": (c_int, [c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_double,
+                                          c_double, _P(c_vp)]),
     "fdadi_mlp_destroy": (c_int, [c_vp]),
     "fdadi_mlp_get_params": (c_int, [c_vp, c_vp, c_vp]),
     "fdadi_mlp_set_params": (c_int, [c_vp, c_vp, c_vp]),

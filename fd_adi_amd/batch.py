@@ -118,6 +118,14 @@ class HestonBatch(object):
         self.scheme = code
         return self
 
+    def configure_dev(self, scheme, theta, dt_scale):
+        """configure() with theta and the dt multiplier in device memory
+        (SizedArrayPtr of one value each); nothing is read back."""
+        code = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        _lib.call("fdadi_hb_configure_dev", self._h, code, theta.p, dt_scale.p, self.stream)
+        self.scheme = code
+        return self
+
     def set_shape(self, kind, threads, lines=0):
         """Launch shape of a sweep kind (0 = spot: threads per work-group and
         ``lines`` = variance columns per tile, 4 or 16; 1 = variance: lanes per

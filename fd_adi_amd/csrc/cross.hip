@@ -76,11 +76,12 @@ __global__ void cross_apply_kernel(const double *__restrict__ bands, const doubl
   y[r] = acc + R[r];
 }
 
+// Scale by vec tiled over the rows, or by the scalar `value` when vec is null.
 __global__ void cross_scale_kernel(double *bands, double *R, const double *__restrict__ vec,
-                                   int64_t vsize, int64_t N) {
+                                   double value, int64_t vsize, int64_t N) {
   const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= N) return;
-  const double s = vec[r % vsize];
+  const double s = vec ? vec[r % vsize] : value;
   for (int k = 0; k < 9; ++k) bands[k * N + r] *= s;
   R[r] *= s;
 }
@@ -266,25 +267,18 @@ int fdadi_cross_scale(fdadi_cross *op, const double *d_vec, int64_t vsize, void 
   FDADI_REQUIRE(vsize > 0 && op->N % vsize == 0, FDADI_ERR_ARG,
                 "Vector length does not divide evenly into operator size. Cannot scale.");
   hipLaunchKernelGGL(cross_scale_kernel, dim3(grid_for(op->N, kBlock)), dim3(kBlock), 0,
-                     as_stream(stream), op->bands, op->R, d_vec, vsize, op->N);
+                     as_stream(stream), op->bands, op->R, d_vec, 0.0, vsize, op->N);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
 
 int fdadi_cross_scale_scalar(fdadi_cross *op, double value, void *stream) {
   FDADI_REQUIRE(op, FDADI_ERR_ARG, "fdadi_cross_scale_scalar: null operator");
-  double *d_v = nullptr;
-  FDADI_TRY_HIP(hipMalloc((void **)&d_v, sizeof(double)));
-  hipError_t e = hipMemcpy(d_v, &value, sizeof(double), hipMemcpyHostToDevice);
-  int st = FDADI_OK;
-  if (e == hipSuccess) st = fdadi_cross_scale(op, d_v, 1, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));
-  (void)hipFree(d_v);
-  if (e != hipSuccess) {
-    set_error(std::string("fdadi_cross_scale_scalar: ") + hipGetErrorString(e));
-    return FDADI_ERR_HIP;
-  }
-  return st;
+  // the scalar travels as a kernel argument: no allocation, copy or sync
+  hipLaunchKernelGGL(cross_scale_kernel, dim3(grid_for(op->N, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), op->bands, op->R, nullptr, value, 1, op->N);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
 }
 
 int fdadi_heston_coefficients(int which, const double *d_spots, int64_t nS,

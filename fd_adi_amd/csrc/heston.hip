@@ -1075,6 +1075,26 @@ __global__ void spot_factor_kernel(Ctx c, const Tile *__restrict__ tiles, int nt
   }
 }
 
+// Step sizes of every option from the scheme's theta and the dt multiplier,
+// taken from device scalars (d_theta, d_scale) when given, else from the host
+// values; the pair used is kept in `slots` for later refactorisations.
+__global__ void step_scalars_kernel(HOpt *opts, int nopt, const double *__restrict__ base_dt,
+                                    const double *d_theta, const double *d_scale, double theta,
+                                    double scale, int implicit, double *slots) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nopt) return;
+  const double th = d_theta ? *d_theta : theta;
+  const double sc = d_scale ? *d_scale : scale;
+  HOpt &o = opts[k];
+  o.dt = base_dt[k] * sc;
+  o.tdt = implicit ? o.dt : th * o.dt;
+  o.th = implicit ? 1.0 : th;
+  if (k == 0 && slots) {
+    slots[0] = th;
+    slots[1] = sc;
+  }
+}
+
 // One work-group per spot row of every option: V(i, j) = max(S_i - K_k, 0).
 __global__ void call_payoff_kernel(const HOpt *opts, const int64_t *row_opt,
                                    const double *__restrict__ spots, int64_t sp_n,
@@ -1156,6 +1176,8 @@ struct fdadi_hb {
   int64_t *d_sp_row = nullptr, *d_vt_row = nullptr;
   Tile *d_stiles = nullptr, *d_vtiles = nullptr, *d_ftiles = nullptr;
   int nstiles = 0, nvtiles = 0, nftiles = 0;
+  int64_t cap_stiles = 0, cap_vtiles = 0;  // tile-list capacity (every launch shape fits)
+  fdadi_arena *arena = nullptr;            // the one device allocation behind every buffer
   int spot_threads = 0;
   int var_tpl = 1, var_rows = 256;  // variance sweep: lanes per line, rows per 256-thread tile
   std::vector<int> order;           // options, largest first
@@ -1163,6 +1185,9 @@ struct fdadi_hb {
   int spot_nl = 4;    // columns per spot-sweep tile (4 or 16)
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
+  bool dev_scalars = false;        // last configure took theta and dt from device memory
+  double *d_base_dt = nullptr;     // each option's step before the multiplier
+  double *d_slots = nullptr;       // [theta, dt multiplier] of the configured scheme
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<hipEvent_t> events;  // pool for fdadi_hb_step_timed
@@ -1236,9 +1261,8 @@ int build_spot_tiles(fdadi_hb *hb) {
     const HOpt &o = hb->h_opts[k];
     for (int64_t j0 = 0; j0 < o.nv; j0 += hb->spot_nl) st.push_back(Tile{k, (int32_t)j0});
   }
-  if (hb->d_stiles) FDADI_TRY_HIP(hipFree(hb->d_stiles));
-  hb->d_stiles = nullptr;
-  FDADI_TRY_HIP(hipMalloc((void **)&hb->d_stiles, sizeof(Tile) * st.size()));
+  FDADI_REQUIRE((int64_t)st.size() <= hb->cap_stiles, FDADI_ERR_STATE,
+                "spot tile list exceeds its capacity");
   FDADI_TRY_HIP(hipMemcpy(hb->d_stiles, st.data(), sizeof(Tile) * st.size(),
                           hipMemcpyHostToDevice));
   hb->nstiles = (int)st.size();
@@ -1268,9 +1292,8 @@ int build_var_tiles(fdadi_hb *hb) {
     const HOpt &o = hb->h_opts[k];
     for (int64_t i0 = 0; i0 < o.nS; i0 += hb->var_rows) vtl.push_back(Tile{k, (int32_t)i0});
   }
-  if (hb->d_vtiles) FDADI_TRY_HIP(hipFree(hb->d_vtiles));
-  hb->d_vtiles = nullptr;
-  FDADI_TRY_HIP(hipMalloc((void **)&hb->d_vtiles, sizeof(Tile) * vtl.size()));
+  FDADI_REQUIRE((int64_t)vtl.size() <= hb->cap_vtiles, FDADI_ERR_STATE,
+                "variance tile list exceeds its capacity");
   FDADI_TRY_HIP(hipMemcpy(hb->d_vtiles, vtl.data(), sizeof(Tile) * vtl.size(),
                           hipMemcpyHostToDevice));
   hb->nvtiles = (int)vtl.size();
@@ -1423,26 +1446,28 @@ int build_tables(fdadi_hb *hb, hipStream_t s) {
 }
 
 // Step sizes of the configured scheme and the time-invariant factorisations
-// of I - tdt A0 and I - tdt A1, on the device; synchronises.
-int refactor(fdadi_hb *hb, hipStream_t s) {
-  for (int k = 0; k < hb->nopt; ++k) {
-    HOpt &o = hb->h_opts[k];
-    o.dt = hb->base_dt[k] * hb->dt_scale;
-    o.tdt = hb->scheme == FDADI_SCHEME_IMPLICIT ? o.dt : hb->theta * o.dt;
-    o.th = hb->scheme == FDADI_SCHEME_IMPLICIT ? 1.0 : hb->theta;
-  }
+// of I - tdt A0 and I - tdt A1, on the device.  theta and the dt multiplier
+// come from the device scalars d_theta / d_scale when given (nothing is read
+// back to the host and nothing synchronises), else from hb->theta /
+// hb->dt_scale (synchronises).
+int refactor(fdadi_hb *hb, hipStream_t s, const double *d_theta = nullptr,
+             const double *d_scale = nullptr) {
+  const bool dev = d_theta || d_scale;
   FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
                                hipMemcpyHostToDevice, s));
+  const bool reuse = d_theta == hb->d_slots;  // refactorisation with the kept pair
+  hipLaunchKernelGGL(step_scalars_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0, s,
+                     hb->d_opts, hb->nopt, hb->d_base_dt, d_theta, d_scale, hb->theta,
+                     hb->dt_scale, (int)(hb->scheme == FDADI_SCHEME_IMPLICIT),
+                     reuse ? nullptr : hb->d_slots);
+  FDADI_CHECK_LAUNCH();
   hipLaunchKernelGGL(var_factor_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0, s, hb->d_opts,
                      hb->nopt, hb->d_vars, hb->d_vt, hb->vt_n);
   FDADI_CHECK_LAUNCH();
   hipLaunchKernelGGL(spot_factor_kernel, dim3(hb->nftiles), dim3(64), 0, s, ctx_of(hb),
                      hb->d_ftiles, hb->nftiles, hb->d_rd0);
   FDADI_CHECK_LAUNCH();
-  // Keep the host copy in step with the factor constants written on the device.
-  FDADI_TRY_HIP(hipMemcpyAsync(hb->h_opts.data(), hb->d_opts, sizeof(HOpt) * hb->nopt,
-                               hipMemcpyDeviceToHost, s));
-  FDADI_TRY_HIP(hipStreamSynchronize(s));
+  if (!dev) FDADI_TRY_HIP(hipStreamSynchronize(s));
   return FDADI_OK;
 }
 
@@ -1451,11 +1476,7 @@ void free_hb(fdadi_hb *hb) {
   if (hb->gexec) (void)hipGraphExecDestroy(hb->gexec);
   if (hb->graph) (void)hipGraphDestroy(hb->graph);
   for (hipEvent_t e : hb->events) (void)hipEventDestroy(e);
-  void *ptrs[] = {hb->d_opts, hb->d_spots, hb->d_vars, hb->d_sp, hb->d_vt, hb->d_spr, hb->d_state,
-                  hb->d_w1, hb->d_y2, hb->d_rd0, hb->d_sp_row, hb->d_vt_row, hb->d_stiles,
-                  hb->d_vtiles, hb->d_ftiles};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
+  (void)fdadi_arena_destroy(hb->arena);
   delete hb;
 }
 
@@ -1535,24 +1556,51 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   hb->order = order;
   hb->var_tpl = var_tpl_for(hb->nvmax);
   hb->var_rows = 256 / hb->var_tpl;
-  hipError_t e = hipSuccess;
-  auto dalloc = [&](void **p, size_t bytes) {
-    if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 8);
+  // Every device buffer of the engine is carved from one arena (one
+  // hipMalloc); the tile lists get the capacity of their largest launch
+  // shape (4-column spot tiles, variance tiles of 4 rows) so that changing
+  // shapes later allocates nothing.
+  for (int k = 0; k < nopt; ++k) {
+    hb->cap_stiles += (hb->h_opts[k].nv + 3) / 4;
+    hb->cap_vtiles += (hb->h_opts[k].nS + 3) / 4;
+  }
+  struct Part {
+    void **p;
+    int64_t bytes;
   };
-  dalloc((void **)&hb->d_opts, sizeof(HOpt) * nopt);
-  dalloc((void **)&hb->d_spots, sizeof(double) * so);
-  dalloc((void **)&hb->d_vars, sizeof(double) * vo);
-  dalloc((void **)&hb->d_sp, sizeof(double) * SP_NF * so);
-  dalloc((void **)&hb->d_vt, sizeof(double) * VT_NF * vo);
-  dalloc((void **)&hb->d_spr, sizeof(double) * 16 * so);
-  dalloc((void **)&hb->d_state, sizeof(double) * off);
-  dalloc((void **)&hb->d_w1, sizeof(double) * off);
-  dalloc((void **)&hb->d_y2, sizeof(double) * off);
-  dalloc((void **)&hb->d_rd0, sizeof(double) * off);
-  dalloc((void **)&hb->d_sp_row, sizeof(int64_t) * so);
-  dalloc((void **)&hb->d_vt_row, sizeof(int64_t) * vo);
-  dalloc((void **)&hb->d_stiles, sizeof(Tile) * st.size());
-  dalloc((void **)&hb->d_ftiles, sizeof(Tile) * ft.size());
+  const Part parts[] = {
+      {(void **)&hb->d_opts, (int64_t)sizeof(HOpt) * nopt},
+      {(void **)&hb->d_spots, 8 * so},
+      {(void **)&hb->d_vars, 8 * vo},
+      {(void **)&hb->d_sp, 8 * SP_NF * so},
+      {(void **)&hb->d_vt, 8 * VT_NF * vo},
+      {(void **)&hb->d_spr, 8 * 16 * so},
+      {(void **)&hb->d_state, 8 * off},
+      {(void **)&hb->d_w1, 8 * off},
+      {(void **)&hb->d_y2, 8 * off},
+      {(void **)&hb->d_rd0, 8 * off},
+      {(void **)&hb->d_sp_row, 8 * so},
+      {(void **)&hb->d_vt_row, 8 * vo},
+      {(void **)&hb->d_stiles, (int64_t)sizeof(Tile) * hb->cap_stiles},
+      {(void **)&hb->d_vtiles, (int64_t)sizeof(Tile) * hb->cap_vtiles},
+      {(void **)&hb->d_ftiles, (int64_t)sizeof(Tile) * (int64_t)ft.size()},
+      {(void **)&hb->d_base_dt, 8 * (int64_t)nopt},
+      {(void **)&hb->d_slots, 16},
+  };
+  int64_t total_bytes = 0;
+  for (const Part &q : parts) total_bytes += (q.bytes + 255) / 256 * 256;
+  if (fdadi_arena_create(total_bytes, &hb->arena) != FDADI_OK) {
+    const std::string msg = fdadi_last_error();
+    free_hb(hb);
+    set_error("fdadi_hb_create: " + msg);
+    return FDADI_ERR_HIP;
+  }
+  for (const Part &q : parts) {
+    double *d = nullptr;
+    (void)fdadi_arena_alloc(hb->arena, (q.bytes + 7) / 8, &d);  // fits by construction
+    *q.p = d;
+  }
+  hipError_t e = hipSuccess;
   auto up = [&](void *dst, const void *src, size_t bytes) {
     if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   };
@@ -1563,6 +1611,7 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   up(hb->d_vt_row, vt_row.data(), sizeof(int64_t) * vo);
   up(hb->d_stiles, st.data(), sizeof(Tile) * st.size());
   up(hb->d_ftiles, ft.data(), sizeof(Tile) * ft.size());
+  up(hb->d_base_dt, hb->base_dt.data(), sizeof(double) * nopt);
   if (e == hipSuccess && build_var_tiles(hb) != FDADI_OK) e = hipErrorUnknown;
   if (e == hipSuccess && build_tables(hb, s) != FDADI_OK) e = hipErrorUnknown;
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1592,7 +1641,31 @@ int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale, 
   hb->scheme = scheme;
   hb->theta = theta;
   hb->dt_scale = dt_scale;
+  hb->dev_scalars = false;
   FDADI_RET(refactor(hb, s));
+  if (!keep_graph) {
+    if (hb->gexec) {
+      (void)hipGraphExecDestroy(hb->gexec);
+      hb->gexec = nullptr;
+    }
+    if (hb->graph) {
+      (void)hipGraphDestroy(hb->graph);
+      hb->graph = nullptr;
+    }
+  }
+  return FDADI_OK;
+}
+
+int fdadi_hb_configure_dev(fdadi_hb *hb, int scheme, const double *d_theta,
+                           const double *d_dt_scale, void *stream) {
+  FDADI_REQUIRE(hb && d_theta && d_dt_scale, FDADI_ERR_ARG,
+                "fdadi_hb_configure_dev: bad arguments");
+  FDADI_REQUIRE(scheme >= FDADI_SCHEME_IMPLICIT && scheme <= FDADI_SCHEME_HV, FDADI_ERR_ARG,
+                "fdadi_hb_configure_dev: unknown scheme");
+  const bool keep_graph = scheme == hb->scheme;
+  hb->scheme = scheme;
+  hb->dev_scalars = true;
+  FDADI_RET(refactor(hb, as_stream(stream), d_theta, d_dt_scale));
   if (!keep_graph) {
     if (hb->gexec) {
       (void)hipGraphExecDestroy(hb->gexec);
@@ -1635,8 +1708,15 @@ int fdadi_hb_update(fdadi_hb *hb, int32_t nopt, const fdadi_heston_option *opts,
                           hipMemcpyHostToDevice));
   FDADI_TRY_HIP(hipMemcpy(hb->d_spots, spots, sizeof(double) * hb->sp_n, hipMemcpyHostToDevice));
   FDADI_TRY_HIP(hipMemcpy(hb->d_vars, vars, sizeof(double) * hb->vt_n, hipMemcpyHostToDevice));
+  FDADI_TRY_HIP(hipMemcpy(hb->d_base_dt, hb->base_dt.data(), sizeof(double) * nopt,
+                          hipMemcpyHostToDevice));
   FDADI_RET(build_tables(hb, s));
-  if (hb->scheme >= 0) FDADI_RET(refactor(hb, s));
+  if (hb->scheme >= 0) {
+    if (hb->dev_scalars)
+      FDADI_RET(refactor(hb, s, hb->d_slots, hb->d_slots + 1));
+    else
+      FDADI_RET(refactor(hb, s));
+  }
   FDADI_TRY_HIP(hipStreamSynchronize(s));
   return FDADI_OK;
 }

@@ -30,6 +30,7 @@ namespace fdadi {
 namespace {
 
 constexpr int kIn = 8;       // Heston features
+constexpr int kMaxIn = 16;   // largest feature count of a network (fdadi_mlp_create_features)
 constexpr int kInPad = 128;  // padded input width
 constexpr int kH = 512;      // hidden width
 constexpr int kLayers = 4;   // hidden layers (the 512 -> 1 head is separate)
@@ -166,9 +167,9 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
 }
 
 // Features -> padded, normalised bf16 input rows (and their transpose):
-// x_f = 2 (p_f - lo_f) / (hi_f - lo_f) - 1 for the 8 Heston parameters.
+// x_f = 2 (p_f - lo_f) / (hi_f - lo_f) - 1 for the nin features of a row.
 __global__ void prep_input_kernel(const double *__restrict__ P, const int *__restrict__ idx,
-                                  int rows, const double *__restrict__ lo,
+                                  int rows, int nin, const double *__restrict__ lo,
                                   const double *__restrict__ hi, bf16 *X, bf16 *XT, int Mpad) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= Mpad) return;
@@ -176,7 +177,8 @@ __global__ void prep_input_kernel(const double *__restrict__ P, const int *__res
   const int src = valid ? (idx ? idx[r] : r) : 0;
   for (int f = 0; f < kInPad; ++f) {
     float v = 0.f;
-    if (valid && f < kIn) v = (float)(2.0 * (P[(size_t)src * kIn + f] - lo[f]) / (hi[f] - lo[f]) - 1.0);
+    if (valid && f < nin)
+      v = (float)(2.0 * (P[(size_t)src * nin + f] - lo[f]) / (hi[f] - lo[f]) - 1.0);
     const bf16 b = static_cast<bf16>(v);
     X[(size_t)r * kInPad + f] = b;
     if (XT) XT[(size_t)f * Mpad + r] = b;
@@ -341,7 +343,8 @@ struct fdadi_mlp {
   bf16 *dZ = nullptr, *dZT = nullptr, *dZ2 = nullptr, *dZT2 = nullptr;
   float *y = nullptr, *dy = nullptr, *loss = nullptr;
   float *wpart = nullptr;  // split-K partial weight gradients [kMaxSplits x kH x kH]
-  double *lohi = nullptr;  // [lo(8) | hi(8)]
+  int n_in = kIn;          // features per input row
+  double *lohi = nullptr;  // [lo(kMaxIn) | hi(kMaxIn)]
   double tmean = 0.0, tstd = 1.0;
   int64_t step = 0;
 };
@@ -407,11 +410,20 @@ extern "C" {
 int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
                      const double *lo, const double *hi, double tmean, double tstd,
                      fdadi_mlp **out) {
+  return fdadi_mlp_create_features(kIn, max_rows, init_params, nparam, lo, hi, tmean, tstd, out);
+}
+
+int fdadi_mlp_create_features(int32_t n_in, int32_t max_rows, const float *init_params,
+                              int64_t nparam, const double *lo, const double *hi, double tmean,
+                              double tstd, fdadi_mlp **out) {
   FDADI_REQUIRE(out && init_params && lo && hi, FDADI_ERR_ARG, "fdadi_mlp_create: null argument");
+  FDADI_REQUIRE(n_in >= 1 && n_in <= kMaxIn, FDADI_ERR_ARG,
+                "fdadi_mlp_create: 1 to 16 input features");
   FDADI_REQUIRE(max_rows > 0 && max_rows % kTile == 0, FDADI_ERR_ARG,
                 "fdadi_mlp_create: max_rows must be a positive multiple of 128");
   fdadi_mlp *m = new fdadi_mlp();
   m->max_rows = max_rows;
+  m->n_in = n_in;
   int64_t o = 0;
   for (int l = 0; l < kLayers; ++l) {
     m->off_W[l] = o;
@@ -459,11 +471,11 @@ int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
   al((void **)&m->dy, sizeof(float) * R);
   al((void **)&m->loss, sizeof(float));
   al((void **)&m->wpart, sizeof(float) * kMaxSplits * kH * kH);
-  al((void **)&m->lohi, sizeof(double) * 2 * kIn);
+  al((void **)&m->lohi, sizeof(double) * 2 * kMaxIn);
   if (e == hipSuccess) e = hipMemcpy(m->p, init_params, sizeof(float) * o, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(m->lohi, lo, sizeof(double) * kIn, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->lohi, lo, sizeof(double) * n_in, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(m->lohi + kIn, hi, sizeof(double) * kIn, hipMemcpyHostToDevice);
+    e = hipMemcpy(m->lohi + kMaxIn, hi, sizeof(double) * n_in, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(zero_kernel, dim3(grid_for(o, 256)), dim3(256), 0, 0, m->m, o);
     hipLaunchKernelGGL(zero_kernel, dim3(grid_for(o, 256)), dim3(256), 0, 0, m->v, o);
@@ -530,7 +542,7 @@ static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const i
                 "fdadi_mlp: batch rows exceed the model capacity");
   const int Mpad = (rows + kTile - 1) / kTile * kTile;
   hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s, d_P, d_idx,
-                     rows, m->lohi, m->lohi + kIn, m->X, m->XT, Mpad);
+                     rows, m->n_in, m->lohi, m->lohi + kMaxIn, m->X, m->XT, Mpad);
   FDADI_CHECK_LAUNCH();
   FDADI_RET(forward(m, Mpad, true, s));
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, m->loss, (int64_t)1);
@@ -617,7 +629,8 @@ int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
     const int rows = (int)std::min<int64_t>(m->max_rows, n - r0);
     const int Mpad = (rows + kTile - 1) / kTile * kTile;
     hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s,
-                       d_P + r0 * kIn, (const int *)nullptr, rows, m->lohi, m->lohi + kIn, m->X,
+                       d_P + r0 * m->n_in, (const int *)nullptr, rows, m->n_in, m->lohi,
+                       m->lohi + kMaxIn, m->X,
                        (bf16 *)nullptr, Mpad);
     FDADI_CHECK_LAUNCH();
     FDADI_RET(forward(m, Mpad, false, s));

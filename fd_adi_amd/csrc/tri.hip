@@ -10,8 +10,9 @@
 //   axis 0: pos * blocks + line    (spot lines, stride nv: coalesced across lines)
 //   axis 1: line * block_len + pos (variance lines, contiguous)
 // The per-line solve is a Thomas sweep with no pivoting (as cuSPARSE gtsv),
-// one thread per line; the +-2 band entries of CAN_FOLD operators are folded
-// on the fly, FOLDED operators fold the right-hand side.
+// one thread per line with the line data staged through LDS for coalesced
+// accesses; the +-2 band entries of CAN_FOLD operators are folded on the
+// fly, FOLDED operators fold the right-hand side.
 #include "common.h"
 
 struct fdadi_tri {
@@ -96,75 +97,180 @@ __global__ void tri_apply_kernel(TriView o, const double *__restrict__ x,
   y[sidx(o, line, pos)] = v;
 }
 
-// One thread per line: Thomas sweep, modified super-diagonal in `cp`, the
-// forward-substituted RHS kept in place in v.
-__global__ void tri_solve_kernel(TriView o, double *__restrict__ v, double *__restrict__ cp) {
-  const int64_t line = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (line >= o.blocks) return;
-  const int64_t n = o.bl, base = line * n;
-  auto rhs = [&](int64_t p) {
+// Thomas sweep (no pivoting), one thread per line, one wave per 64
+// consecutive lines.  The bands, the right-hand side, the modified
+// super-diagonal (`cp`, in scratch) and the forward-substituted values
+// (kept in place in v) move between global memory and LDS in tiles of 64
+// lines x kSolveChunk positions with coalesced accesses -- 128-byte runs of
+// one line for line-major arrays, 64 consecutive lines of one position for
+// the strided state of axis 0 -- and each thread runs its line's
+// recurrences out of the LDS tile (odd pitch: no bank conflicts).  Rows 0
+// and n-1 (with their fold entries) are read directly, once per line.
+constexpr int kSolveChunk = 16;
+constexpr int kSolvePitch = kSolveChunk + 1;
+
+__device__ __forceinline__ void tile_index(const TriView &o, int64_t L0, int64_t p0, int e,
+                                           bool state, int &l, int &p, int64_t &g, bool &ok) {
+  const int64_t n = o.bl;
+  if (state && o.axis == 0) {  // position-major: 64 lines of one position per load
+    p = e / 64;
+    l = e % 64;
+  } else {  // line-major: runs of kSolveChunk positions of one line
+    l = e / kSolveChunk;
+    p = e % kSolveChunk;
+  }
+  const int64_t line = L0 + l, pos = p0 + p;
+  ok = line < o.blocks && pos < n;
+  g = ok ? (state ? sidx(o, line, pos) : line * n + pos) : 0;
+}
+
+__global__ void __launch_bounds__(64)
+tri_solve_kernel(TriView o, double *__restrict__ v, double *__restrict__ cpg) {
+  __shared__ double ta[64 * kSolvePitch], tb[64 * kSolvePitch], tc[64 * kSolvePitch],
+      td[64 * kSolvePitch];
+  const int t = threadIdx.x;
+  const int64_t L0 = (int64_t)blockIdx.x * 64, line = L0 + t, n = o.bl, base = line * n;
+  const bool live = line < o.blocks;
+  auto rhs_at = [&](int64_t p) {
     double d = xf(o, v, line, p);
     if (o.has_residual) d -= o.R[base + p];
     return d;
   };
-  // Row 0 (with top fold) and the last row (with bottom fold) are special.
-  double b0 = o.mid[base], c0 = o.sup[base], d0 = rhs(0);
-  if (o.top_status == FDADI_FOLDED) {
-    d0 += o.top[line] * rhs(1);
-  } else if (o.top_status == FDADI_CAN_FOLD) {
-    const double c1 = o.sup[base + 1];
-    const double f = c1 == 0 ? 0.0 : -o.top[line] / c1;
-    b0 += f * o.sub[base + 1];
-    c0 += f * o.mid[base + 1];
-    d0 += f * rhs(1);
-  }
-  double an = o.sub[base + n - 1], bn = o.mid[base + n - 1];
-  double fbot = 0.0;
-  if (o.bottom_status == FDADI_FOLDED) {
-    fbot = o.bottom[line];
-  } else if (o.bottom_status == FDADI_CAN_FOLD) {
-    const double a = o.sub[base + n - 2];
-    fbot = a == 0 ? 0.0 : -o.bottom[line] / a;
-    an += fbot * o.mid[base + n - 2];
-    bn += fbot * o.sup[base + n - 2];
-  }
-  // Forward elimination.
-  double cprev = c0 / b0;
-  double dprev = d0 / b0;
-  cp[base] = cprev;
-  v[sidx(o, line, 0)] = dprev;
-  double dnm2 = 0.0;  // raw rhs of row n-2 (for the bottom fold)
-  for (int64_t p = 1; p < n; ++p) {
-    double a, b, c, d;
-    if (p == n - 1) {
-      d = rhs(p) + fbot * dnm2;
-      a = an; b = bn; c = 0.0;
-    } else {
-      d = rhs(p);
-      if (p == n - 2) dnm2 = d;
-      a = o.sub[base + p]; b = o.mid[base + p]; c = o.sup[base + p];
+  // Row 0 (with top fold) and the last row (with bottom fold), per line.
+  double b0 = 0.0, c0 = 0.0, d0 = 0.0, an = 0.0, bn = 0.0, fbot = 0.0;
+  if (live) {
+    b0 = o.mid[base];
+    c0 = o.sup[base];
+    d0 = rhs_at(0);
+    if (o.top_status == FDADI_FOLDED) {
+      d0 += o.top[line] * rhs_at(1);
+    } else if (o.top_status == FDADI_CAN_FOLD) {
+      const double c1 = o.sup[base + 1];
+      const double f = c1 == 0 ? 0.0 : -o.top[line] / c1;
+      b0 += f * o.sub[base + 1];
+      c0 += f * o.mid[base + 1];
+      d0 += f * rhs_at(1);
     }
-    const double m = 1.0 / (b - a * cprev);
-    cprev = c * m;
-    dprev = (d - a * dprev) * m;
-    cp[base + p] = cprev;
-    v[sidx(o, line, p)] = dprev;
+    an = o.sub[base + n - 1];
+    bn = o.mid[base + n - 1];
+    if (o.bottom_status == FDADI_FOLDED) {
+      fbot = o.bottom[line];
+    } else if (o.bottom_status == FDADI_CAN_FOLD) {
+      const double a = o.sub[base + n - 2];
+      fbot = a == 0 ? 0.0 : -o.bottom[line] / a;
+      an += fbot * o.mid[base + n - 2];
+      bn += fbot * o.sup[base + n - 2];
+    }
   }
-  // Back substitution.
+  // Forward elimination, chunk by chunk.
+  double cprev = 0.0, dprev = 0.0, dnm2 = 0.0;
+  for (int64_t p0 = 0; p0 < n; p0 += kSolveChunk) {
+    for (int e = t; e < 64 * kSolveChunk; e += 64) {
+      int l, p;
+      int64_t g;
+      bool ok;
+      tile_index(o, L0, p0, e, false, l, p, g, ok);
+      ta[l * kSolvePitch + p] = ok ? o.sub[g] : 0.0;
+      tb[l * kSolvePitch + p] = ok ? o.mid[g] : 0.0;
+      tc[l * kSolvePitch + p] = ok ? o.sup[g] : 0.0;
+      if (o.has_residual) td[l * kSolvePitch + p] = ok ? -o.R[g] : 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * kSolveChunk; e += 64) {
+      int l, p;
+      int64_t g;
+      bool ok;
+      tile_index(o, L0, p0, e, true, l, p, g, ok);
+      const int64_t ln = L0 + l, pos = p0 + p;
+      double x = 0.0;
+      if (ok) {
+        if (pos == 0 && o.has_low) x = o.low[ln];
+        else if (pos == n - 1 && o.has_high) x = o.high[ln];
+        else x = v[g];
+      }
+      // rhs = x - R (the R tile holds -R when the residual is on)
+      double &slot = td[l * kSolvePitch + p];
+      slot = o.has_residual ? x + slot : x;
+    }
+    __syncthreads();
+    if (live) {
+      for (int k = 0; k < kSolveChunk && p0 + k < n; ++k) {
+        const int64_t pos = p0 + k;
+        double *ck = &tc[t * kSolvePitch + k], *dk = &td[t * kSolvePitch + k];
+        if (pos == 0) {
+          cprev = c0 / b0;
+          dprev = d0 / b0;
+        } else {
+          double a, b, c, d;
+          if (pos == n - 1) {
+            d = *dk + fbot * dnm2;
+            a = an; b = bn; c = 0.0;
+          } else {
+            d = *dk;
+            if (pos == n - 2) dnm2 = d;
+            a = ta[t * kSolvePitch + k]; b = tb[t * kSolvePitch + k]; c = *ck;
+          }
+          const double m = 1.0 / (b - a * cprev);
+          cprev = c * m;
+          dprev = (d - a * dprev) * m;
+        }
+        *ck = cprev;
+        *dk = dprev;
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * kSolveChunk; e += 64) {
+      int l, p;
+      int64_t g;
+      bool ok;
+      tile_index(o, L0, p0, e, false, l, p, g, ok);
+      if (ok) cpg[g] = tc[l * kSolvePitch + p];
+      tile_index(o, L0, p0, e, true, l, p, g, ok);
+      if (ok) v[g] = td[l * kSolvePitch + p];
+    }
+    __syncthreads();
+  }
+  // Back substitution, chunks from the last; x(n-1) = dprev.
   double x = dprev;
-  for (int64_t p = n - 2; p >= 0; --p) {
-    const int64_t s = sidx(o, line, p);
-    x = v[s] - cp[base + p] * x;
-    v[s] = x;
+  const int64_t last0 = (n - 1) / kSolveChunk * kSolveChunk;
+  for (int64_t p0 = last0; p0 >= 0; p0 -= kSolveChunk) {
+    for (int e = t; e < 64 * kSolveChunk; e += 64) {
+      int l, p;
+      int64_t g;
+      bool ok;
+      tile_index(o, L0, p0, e, false, l, p, g, ok);
+      tc[l * kSolvePitch + p] = ok ? cpg[g] : 0.0;
+      tile_index(o, L0, p0, e, true, l, p, g, ok);
+      td[l * kSolvePitch + p] = ok ? v[g] : 0.0;
+    }
+    __syncthreads();
+    if (live) {
+      for (int k = kSolveChunk - 1; k >= 0; --k) {
+        const int64_t pos = p0 + k;
+        if (pos > n - 2) continue;  // x(n-1) is dprev, already in place
+        x = td[t * kSolvePitch + k] - tc[t * kSolvePitch + k] * x;
+        td[t * kSolvePitch + k] = x;
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * kSolveChunk; e += 64) {
+      int l, p;
+      int64_t g;
+      bool ok;
+      tile_index(o, L0, p0, e, true, l, p, g, ok);
+      if (ok) v[g] = td[l * kSolvePitch + p];
+    }
+    __syncthreads();
   }
 }
 
+// Scale by vec tiled over the rows, or by the scalar `value` when vec is null.
 __global__ void tri_scale_kernel(double *sup, double *mid, double *sub, double *R,
-                                 const double *__restrict__ vec, int64_t vsize,
+                                 const double *__restrict__ vec, double value, int64_t vsize,
                                  int64_t rows, int64_t bl, int has_low, int has_high) {
   const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= rows) return;
-  const double s = vec[r % vsize];
+  const double s = vec ? vec[r % vsize] : value;
   const int64_t pos = r % bl;
   sup[r] *= s;
   sub[r] *= s;
@@ -174,13 +280,13 @@ __global__ void tri_scale_kernel(double *sup, double *mid, double *sub, double *
 }
 
 __global__ void tri_scale_factors_kernel(double *top, double *bottom,
-                                         const double *__restrict__ vec, int64_t vsize,
-                                         int64_t blocks, int64_t bl, int scale_top,
-                                         int scale_bottom) {
+                                         const double *__restrict__ vec, double value,
+                                         int64_t vsize, int64_t blocks, int64_t bl,
+                                         int scale_top, int scale_bottom) {
   const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (b >= blocks) return;
-  if (scale_top) top[b] *= vec[(b * bl) % vsize];
-  if (scale_bottom) bottom[b] *= vec[(b * bl + bl - 1) % vsize];
+  if (scale_top) top[b] *= vec ? vec[(b * bl) % vsize] : value;
+  if (scale_bottom) bottom[b] *= vec ? vec[(b * bl + bl - 1) % vsize] : value;
 }
 
 __global__ void tri_add_kernel(double *sup, double *mid, double *sub, double *R,
@@ -645,40 +751,44 @@ int fdadi_tri_add_scalar_dev(fdadi_tri *op, const double *d_value, void *stream)
   return add_scalar_impl(op, d_value, 0.0, stream);
 }
 
-int fdadi_tri_scale(fdadi_tri *op, const double *d_vec, int64_t vsize, void *stream) {
-  FDADI_REQUIRE(op && d_vec, FDADI_ERR_ARG, "fdadi_tri_scale: null argument");
+namespace {
+// vectorized_scale with a device vector (d_vec) or a host scalar (d_vec null).
+int scale_impl(fdadi_tri *op, const double *d_vec, double value, int64_t vsize,
+               void *stream) {
   FDADI_REQUIRE(op->top_status != FDADI_FOLDED && op->bottom_status != FDADI_FOLDED,
                 FDADI_ERR_STATE, "Cannot scale diagonalized operator.");
-  FDADI_REQUIRE(vsize > 0, FDADI_ERR_ARG, "vsize == 0");
-  FDADI_REQUIRE(op->rows % vsize == 0, FDADI_ERR_ARG,
-                "Vector length does not divide evenly into operator size. Cannot scale. "
-                "vsize(" + std::to_string(vsize) + ") operator_rows(" +
-                    std::to_string(op->rows) + ")");
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(tri_scale_kernel, dim3(grid_for(op->rows, kBlock)), dim3(kBlock), 0, s,
-                     op->sup(), op->mid(), op->sub(), op->R, d_vec, vsize, op->rows,
+                     op->sup(), op->mid(), op->sub(), op->R, d_vec, value, vsize, op->rows,
                      op->block_len, op->has_low, op->has_high);
   FDADI_CHECK_LAUNCH();
   const int st = op->top_status == FDADI_CAN_FOLD && !op->has_low;
   const int sb = op->bottom_status == FDADI_CAN_FOLD && !op->has_high;
   if (st || sb) {
     hipLaunchKernelGGL(tri_scale_factors_kernel, dim3(grid_for(op->blocks, kBlock)),
-                       dim3(kBlock), 0, s, op->top(), op->bottom(), d_vec, vsize, op->blocks,
-                       op->block_len, st, sb);
+                       dim3(kBlock), 0, s, op->top(), op->bottom(), d_vec, value, vsize,
+                       op->blocks, op->block_len, st, sb);
     FDADI_CHECK_LAUNCH();
   }
   return FDADI_OK;
 }
+}  // namespace
+
+int fdadi_tri_scale(fdadi_tri *op, const double *d_vec, int64_t vsize, void *stream) {
+  FDADI_REQUIRE(op && d_vec, FDADI_ERR_ARG, "fdadi_tri_scale: null argument");
+  FDADI_REQUIRE(vsize > 0, FDADI_ERR_ARG, "vsize == 0");
+  FDADI_REQUIRE(op->rows % vsize == 0, FDADI_ERR_ARG,
+                "Vector length does not divide evenly into operator size. Cannot scale. "
+                "vsize(" + std::to_string(vsize) + ") operator_rows(" +
+                    std::to_string(op->rows) + ")");
+  return scale_impl(op, d_vec, 0.0, vsize, stream);
+}
 
 int fdadi_tri_scale_scalar(fdadi_tri *op, double value, void *stream) {
   FDADI_REQUIRE(op, FDADI_ERR_ARG, "fdadi_tri_scale_scalar: null operator");
-  // A one-element device vector tiled over all rows (mul_scalar_from_host,
-  // BandedOperatorGPU.pyx:488-501).  The staging copy lives on the stream.
-  double *d_v = op->scratch;  // rows >= 1; scratch is dead between solves
-  FDADI_TRY_HIP(hipMemcpyAsync(d_v, &value, sizeof(double), hipMemcpyHostToDevice,
-                               as_stream(stream)));
-  FDADI_TRY_HIP(hipStreamSynchronize(as_stream(stream)));
-  return fdadi_tri_scale(op, d_v, 1, stream);
+  // mul_scalar_from_host (BandedOperatorGPU.pyx:488-501): the scalar travels
+  // as a kernel argument (no staging copy, no synchronisation).
+  return scale_impl(op, nullptr, value, 1, stream);
 }
 
 int fdadi_tri_fold_top(fdadi_tri *op, int unfold, void *stream) {

@@ -3,6 +3,7 @@
 * ``single(...)``: one European call on a sinh (spot) x power-law (variance)
   grid, built exactly as the engine builds it
   (FiniteDifference/FiniteDifferenceEngineGPU.pyx:658-686).
+* ``label_draws(n, seed)``: surrogate rows on the label meshes (below).
 * ``batch_draws(n, seed, ...)``: the batched workload: per option
   kappa~U[0.5,5], theta~U[0.01,0.16], sigma~U[0.1,1], rho~U[-0.9,0],
   r~U[0,0.08], T~U[0.1,3], K/S0~U[0.7,1.3] (S0=100), v0=theta (or drawn),
@@ -72,3 +73,15 @@ def batch_draws(n, seed, nS_range=(64, 512), nv_range=(32, 256), nt=200, draw_v0
         vars_ = exponential_space(0.0, v0, 5 * max(theta, v0), 4.0, nv)
         out.append(Option1(100.0, K, r, v0, T, kappa, theta, sigma, rho, spots, vars_, nt))
     return out
+
+
+def label_draws(n, seed, nS=256, nv=128, nt=200):
+    """``n`` surrogate parameter rows drawn from ``seed`` as options on the
+    label meshes (spot mesh K x one unit sinh mesh, fixed power-law variance
+    mesh; S0 = 100 and v0 off the nodes): the options the surrogate's labels
+    and its ADI fallback solve."""
+    from .surrogate import draw_params, label_grids
+    P = draw_params(n, seed)
+    S, V = label_grids(P, nS, nv)
+    return [Option1(100.0, 100.0 * p[7], p[5], p[4], p[6], p[0], p[1], p[2], p[3], s, v, nt)
+            for p, s, v in zip(P, S, V)]

@@ -192,6 +192,12 @@ int fdadi_hb_destroy(fdadi_hb *hb);
  * (theta and dt are read from device memory at every step). */
 int fdadi_hb_configure(fdadi_hb *hb, int scheme, double theta, double dt_scale,
                        void *stream);
+/* fdadi_hb_configure with theta and the dt multiplier read from device
+ * memory (one double each), as the reference's device variants take them
+ * (SizedArrayPtr dt / theta, FiniteDifferenceEngineGPU.pyx:379,502,582);
+ * asynchronous: nothing is read back to the host. */
+int fdadi_hb_configure_dev(fdadi_hb *hb, int scheme, const double *d_theta,
+                           const double *d_dt_scale, void *stream);
 /* New parameters, step sizes and meshes for the engine's options, of the
  * same count and grid shapes as at creation (host arrays as for create); the
  * tables are rebuilt and, when configured, the factorisations too, so a
@@ -252,6 +258,12 @@ int fdadi_mlp_nparam(int64_t *n);
 int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
                      const double *lo, const double *hi, double tmean, double tstd,
                      fdadi_mlp **out);
+/* The same network on n_in (1..16) input features per row (d_P rows of n_in
+ * doubles, lo/hi of n_in): the surrogate's error model reads the 8 features
+ * plus derived ones.  fdadi_mlp_create is this with n_in = 8. */
+int fdadi_mlp_create_features(int32_t n_in, int32_t max_rows, const float *init_params,
+                              int64_t nparam, const double *lo, const double *hi, double tmean,
+                              double tstd, fdadi_mlp **out);
 int fdadi_mlp_destroy(fdadi_mlp *m);
 int fdadi_mlp_get_params(const fdadi_mlp *m, float *params, float *grads);
 int fdadi_mlp_set_params(fdadi_mlp *m, const float *params, void *stream);

@@ -221,3 +221,29 @@ def test_unsnapped_payoff_from_engine_domain(fd, scheme):
     npt.assert_array_equal(g.domain[-1], U0)
     V2 = getattr(G2, "solve_" + scheme)(50, 0.02)
     npt.assert_array_equal(V2, V)
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "operators"])
+def test_device_scalar_dt_theta(fd, fused):
+    """The device variants take dt (and theta for HV) as device scalars, as
+    the reference's do (FiniteDifferenceEngineGPU.pyx:379,502,582):
+    solve_implicit_, solve_hundsdorferverwer_ and solve_smooth_ with
+    SizedArrayPtr dt / theta against the oracle; then a host-scalar solve on
+    the same engine (the configuration switches back to host values)."""
+    from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
+    G, O = make(40, 20, -0.4)
+    G.fused = fused
+    U0 = O.payoff()
+    dt, th = SizedArrayPtr(np.array([0.05])), SizedArrayPtr(np.array([0.5]))
+    V = SizedArrayPtr(U0)
+    G.solve_hundsdorferverwer_(10, dt, V, th)
+    check(V.to_numpy(), oracle_both(O, lambda E: E.solve_hundsdorferverwer(10, 0.05, U0)), O.idx)
+    V = SizedArrayPtr(U0)
+    G.solve_implicit_(8, dt, V)
+    check(V.to_numpy(), oracle_both(O, lambda E: E.solve_implicit(8, 0.05, U0)), O.idx)
+    V = SizedArrayPtr(U0)
+    G.solve_smooth_(12, dt, V)
+    check(V.to_numpy(), oracle_both(O, lambda E: E.solve_smooth(12, 0.05, U0)), O.idx)
+    V = G.solve_hundsdorferverwer(10, 0.04, U0, theta=0.6)
+    check(V, oracle_both(O, lambda E: E.solve_hundsdorferverwer(10, 0.04, U0, theta=0.6)),
+          O.idx)

@@ -132,3 +132,24 @@ def test_config4_serving_routes_to_oracle_checked_adi(S, labeler):
     err_acc = np.abs(price[accepted] - want[accepted])
     print("fallback %d of %d (outside the box %d); accepted: max |err| vs oracle %.2e"
           % ((~accepted).sum(), len(Q), (~inside).sum(), err_acc.max()))
+
+
+def test_fitted_surrogate_gate_and_routing(S):
+    """fit_surrogate end to end on a small label set: the error model and
+    the calibrated threshold route the queries -- admitted ones lie inside
+    the box and have score <= tau and get the surrogate's price, every other
+    one (all those outside the box among them) gets the FP64 ADI label."""
+    P = S.draw_params(8192, 306)
+    y = S.adi_prices(P, NS, NV, NT)
+    pricer, info = S.fit_surrogate(P, y, members=2, steps=300, batch=2048, max_fail=0.01)
+    print(info)
+    assert info["error_model"] == "trees" and info["calib_rows"] > 600
+    Q = S.draw_params(512, 307, outside_frac=0.2)
+    mean, spread = pricer.ensemble.predict(Q)
+    score = pricer.error_model(Q, spread)
+    price, accepted = pricer.price(Q)
+    inside = np.all((Q >= S.BOX_LO) & (Q <= S.BOX_HI), axis=1)
+    npt.assert_array_equal(accepted, inside & (score <= pricer.tau))
+    assert not accepted[~inside].any() and pricer.last_fallback == (~accepted).sum()
+    npt.assert_array_equal(price[accepted], mean[accepted])
+    npt.assert_array_equal(price[~accepted], S.adi_prices(Q[~accepted], NS, NV, NT))

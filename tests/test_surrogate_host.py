@@ -79,6 +79,6 @@ def test_gate_rejects_everything_without_a_threshold():
     import pytest
     with pytest.raises(ValueError):
         SurrogatePricer(One(), 0.0)
-    pr = SurrogatePricer(One(), -np.inf, error_model=lambda P: np.zeros(len(P)))
+    pr = SurrogatePricer(One(), -np.inf, error_model=lambda P, s: np.zeros(len(P)))
     assert not pr.gate(P, np.zeros(len(P))).any()
     assert isinstance(Ensemble([None, None]).members, list)
