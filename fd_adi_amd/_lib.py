@@ -120,6 +120,7 @@ _SIGS = {
     "fdadi_mlp_train_step": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, ctypes.c_float, c_vp]),
     "fdadi_mlp_loss": (c_int, [c_vp, c_vp]),
     "fdadi_mlp_predict": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "fdadi_mlp_predict_noalloc": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "fdadi_mlp_gemm_nt": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
     "fdadi_heston_cf_call": (c_int, [c_vp, c_i64, c_vp, c_vp]),
     "fdadi_gauss_legendre": (c_int, [c_i32, c_vp, c_vp]),

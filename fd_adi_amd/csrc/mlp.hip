@@ -13,6 +13,8 @@
 // is a multiple of the 128 x 128 work-group tile.
 #include "common.h"
 
+#include <mutex>
+
 #include <cmath>
 #include <vector>
 
@@ -402,10 +404,100 @@ int forward(fdadi_mlp *m, int Mpad, bool train, hipStream_t s) {
   return FDADI_OK;
 }
 
+// Inference forward pass into caller-given buffers: input X [Mpad x 128],
+// hidden activations alternating between A0 and A1 [Mpad x 512], head y [Mpad].
+int forward_inference(const fdadi_mlp *m, const bf16 *X, bf16 *A0, bf16 *A1, float *y, int Mpad,
+                      hipStream_t s) {
+  const bf16 *in = X;
+  bf16 *bufs[2] = {A0, A1};
+  for (int l = 0; l < kLayers; ++l) {
+    EpiArgs ep{};
+    ep.bias = m->p + m->off_b[l];
+    ep.out = bufs[l & 1];
+    FDADI_RET(gemm(EPI_INF, in, m->Wb[l], Mpad, kH, in_dim(l), ep, s));
+    in = bufs[l & 1];
+  }
+  hipLaunchKernelGGL(head_forward_kernel, dim3((Mpad + 3) / 4), dim3(256), 0, s, in, m->w4b,
+                     m->p + m->off_b4, Mpad, y);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+// ---- allocation-free inference ---------------------------------------------------
+// fdadi_mlp_predict_noalloc keeps its activations in this file-scope device
+// array instead of the model's buffers: X [kNoAllocRows x 128] bf16, two
+// hidden-activation buffers [kNoAllocRows x 512] bf16, the head output
+// [kNoAllocRows] fp32 -- 9 MiB + 16 KiB on every GPU the library runs on,
+// held until the process ends.  In host code `noalloc_scratch` names a
+// host-side stand-in: kernels get the device address resolved per device by
+// hipGetSymbolAddress.
+constexpr int kNoAllocRows = 4096;
+constexpr size_t kNoAllocX = (size_t)kNoAllocRows * kInPad * sizeof(bf16);
+constexpr size_t kNoAllocA = (size_t)kNoAllocRows * kH * sizeof(bf16);
+constexpr size_t kNoAllocY = (size_t)kNoAllocRows * sizeof(float);
+constexpr size_t kNoAllocBytes = kNoAllocX + 2 * kNoAllocA + kNoAllocY;
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+unsigned char *g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of noalloc_scratch on the current device, resolved once per
+// device with that device current.
+int noalloc_scratch_ptr(unsigned char **out) {
+  int dev = 0;
+  FDADI_TRY_HIP(hipGetDevice(&dev));
+  FDADI_REQUIRE(dev >= 0 && dev < kMaxDevices, FDADI_ERR_UNSUPPORTED,
+                "fdadi_mlp_predict_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    FDADI_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    FDADI_REQUIRE(p != nullptr, FDADI_ERR_HIP,
+                  "fdadi_mlp_predict_noalloc: no device address for the scratch array");
+    g_noalloc_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return FDADI_OK;
+}
+
 }  // namespace
 }  // namespace fdadi
 
 extern "C" {
+
+int fdadi_mlp_predict_noalloc(const fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
+                              void *stream) {
+  FDADI_REQUIRE(m && n >= 0 && (n == 0 || (d_P && d_out)), FDADI_ERR_ARG,
+                "fdadi_mlp_predict_noalloc: bad arguments");
+  FDADI_REQUIRE(n <= kNoAllocRows, FDADI_ERR_ARG,
+                "fdadi_mlp_predict_noalloc: " + std::to_string(n) +
+                    " rows exceed the maximum of " + std::to_string(kNoAllocRows) +
+                    " per call (split the batch, or use fdadi_mlp_predict)");
+  if (n == 0) return FDADI_OK;
+  unsigned char *base = nullptr;
+  FDADI_RET(noalloc_scratch_ptr(&base));
+  bf16 *X = reinterpret_cast<bf16 *>(base);
+  bf16 *A0 = reinterpret_cast<bf16 *>(base + kNoAllocX);
+  bf16 *A1 = reinterpret_cast<bf16 *>(base + kNoAllocX + kNoAllocA);
+  float *y = reinterpret_cast<float *>(base + kNoAllocX + 2 * kNoAllocA);
+  hipStream_t s = as_stream(stream);
+  const int rows = (int)n;
+  const int Mpad = (rows + kTile - 1) / kTile * kTile;
+  // every row of X [0, Mpad) is written (padding rows as zeros), every
+  // activation and head row of [0, Mpad) by the GEMMs: nothing carries over
+  // from the previous call
+  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s, d_P,
+                     (const int *)nullptr, rows, m->n_in, m->lohi, m->lohi + kMaxIn, X,
+                     (bf16 *)nullptr,
+                     Mpad);
+  FDADI_CHECK_LAUNCH();
+  FDADI_RET(forward_inference(m, X, A0, A1, y, Mpad, s));
+  hipLaunchKernelGGL(unscale_kernel, dim3(grid_for(rows, 256)), dim3(256), 0, s, y, rows,
+                     m->tmean, m->tstd, d_out);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
 
 int fdadi_mlp_create(int32_t max_rows, const float *init_params, int64_t nparam,
                      const double *lo, const double *hi, double tmean, double tstd,

@@ -277,6 +277,16 @@ int fdadi_mlp_train_step(fdadi_mlp *m, const double *d_P, const double *d_T, con
 int fdadi_mlp_loss(const fdadi_mlp *m, float *loss);
 /* Prices for d_P [n x 8] into d_out [n]. */
 int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out, void *stream);
+/* fdadi_mlp_predict with the activations in a static device array instead
+ * of the model's buffers: allocates nothing (no hipMalloc, no hipFree, no
+ * allocating library call).  No counterpart in the reference: it is for
+ * callers that cannot allocate -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation.  At most 4096 rows per call
+ * (FDADI_ERR_ARG beyond; split the batch).  The array (9 MiB + 16 KiB) lives
+ * on every GPU the library runs on until the process ends and is shared by
+ * all calls on that GPU: calls that overlap on one GPU race. */
+int fdadi_mlp_predict_noalloc(const fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
+                              void *stream);
 /* C (fp32) = A B^T with bf16 operands given as raw bits (test entry point). */
 int fdadi_mlp_gemm_nt(const uint16_t *d_A, const uint16_t *d_B, int32_t M, int32_t N, int32_t K,
                       float *d_C, void *stream);
