@@ -307,8 +307,8 @@ class AdiLabeler(object):
     last row (the extra options are discarded).  With ``tune`` the launch
     shapes of each engine are picked by ``HestonBatch.autotune``."""
 
-    def __init__(self, nS=256, nv=128, nt=200, scheme="cs", theta=0.5, tiers=(256, 1024, 4096),
-                 tune=True):
+    def __init__(self, nS=256, nv=128, nt=200, scheme="cs", theta=0.5,
+                 tiers=(256, 1024, 2048, 4096), tune=True):
         self.nS, self.nv, self.nt = int(nS), int(nv), int(nt)
         self.scheme, self.theta = scheme, float(theta)
         self.tiers = tuple(sorted(int(t) for t in tiers))

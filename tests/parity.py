@@ -6,11 +6,13 @@ the oracle's own rounding noise: the largest change of the oracle's grid
 when only the order of its floating-point operations changes -- its explicit
 operators summed in the opposite order (the reference's order is Python 2
 dict order, i.e. arbitrary), every band product summed in reverse, or the
-tridiagonal systems eliminated by Thomas's algorithm instead of LAPACK.
+tridiagonal systems eliminated by Thomas's algorithm without pivoting
+(variance systems folded first) instead of LAPACK's pivoted band solver.
 Far-field rows (S near Smax, values ~1e3, coefficients ~1e4) make that floor
 exceed 1e-10 on the reference's own grids, so a fixed 1e-10 would reject the
 oracle against itself.  Measured on MI355X over every parity case of the
-suite, the GPU's error is at most 2.3x that floor.
+suite (tests/parity_cases.md, written by tests/parity_table.py), the GPU's
+error, where it exceeds 1e-10, is at most 1.56x that floor.
 """
 
 import json
