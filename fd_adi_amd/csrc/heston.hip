@@ -229,39 +229,49 @@ __device__ __forceinline__ double from_next_lane(double x, int ln) {
 }
 
 // Exclusive scan of affine maps y -> E + M y along the TPL chunks of each of
-// the NL lines of a tile.  Entries live in LDS at [c*NL + l].  Forward
+// the NL lines of a tile.  Entries live in LDS at [l*(TPL+1) + c] (line by
+// line, odd pitch: the lanes of a line read consecutive entries and the
+// writers of one chunk hit distinct banks).  Forward
 // (REV=false): carry(0)=0, carry(c+1)=E[c]+M[c]*carry(c).  Backward
-// (REV=true): the same from the last chunk down.  One wave per line.
+// (REV=true): the same from the last chunk down.  A line takes W = min(TPL,
+// 64) lanes (K = TPL / W entries per lane), so a wave scans 64 / W lines at
+// once with segmented scans.
 template <int THREADS, int NL, bool REV>
-__device__ void affine_scan(const double *cM, const double *cE, double *cIn, int tpl) {
-  constexpr int W = 64;
-  const int lane = threadIdx.x % W, wave = threadIdx.x / W;
-  constexpr int NW = THREADS / W;
-  const int K = (tpl + W - 1) / W;  // entries per lane
-  for (int l = wave; l < NL; l += NW) {
+__device__ void affine_scan(const double *cM, const double *cE, double *cIn) {
+  constexpr int TPL = THREADS / NL;
+  constexpr int TPLP = TPL + 1;
+  constexpr int W = TPL < 64 ? TPL : 64;
+  constexpr int K = TPL / W;
+  constexpr int LPW = 64 / W;
+  constexpr int NW = THREADS / 64;
+  static_assert(W >= 16 && (W & (W - 1)) == 0, "16 to 64 lanes per line");
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int ln = lane % W;
+  for (int l0 = wave * LPW; l0 < NL; l0 += NW * LPW) {
+    const int l = l0 + lane / W;
+    const bool ok = l < NL;
     // Compose this lane's K entries (in scan order).
     double Mloc = 1.0, Eloc = 0.0;
+#pragma unroll
     for (int q = 0; q < K; ++q) {
-      const int s = lane * K + q;
-      if (s < tpl) {
-        const int cidx = REV ? tpl - 1 - s : s;
-        const double m = cM[cidx * NL + l], e = cE[cidx * NL + l];
-        Eloc = e + m * Eloc;
-        Mloc = m * Mloc;
-      }
+      const int s = ln * K + q;
+      const int cidx = REV ? TPL - 1 - s : s;
+      const double m = ok ? cM[l * TPLP + cidx] : 1.0, e = ok ? cE[l * TPLP + cidx] : 0.0;
+      Eloc = e + m * Eloc;
+      Mloc = m * Mloc;
     }
-    // Inclusive wave scan of (M, E) (later maps applied after earlier ones).
+    // Inclusive segmented scan of (M, E) (later maps applied after earlier ones).
     double M = Mloc, E = Eloc;
-    wave_prefix_scan<W>(M, E, lane);
+    wave_prefix_scan<W>(M, E, ln);
     // Exclusive: the carry entering this lane's first entry.
-    const double carry0 = from_prev_lane<W>(E, lane);
-    double carry = carry0;
+    double carry = from_prev_lane<W>(E, ln);
+#pragma unroll
     for (int q = 0; q < K; ++q) {
-      const int s = lane * K + q;
-      if (s < tpl) {
-        const int cidx = REV ? tpl - 1 - s : s;
-        cIn[cidx * NL + l] = carry;
-        carry = cE[cidx * NL + l] + cM[cidx * NL + l] * carry;
+      const int s = ln * K + q;
+      const int cidx = REV ? TPL - 1 - s : s;
+      if (ok) {
+        cIn[l * TPLP + cidx] = carry;
+        carry = cE[l * TPLP + cidx] + cM[l * TPLP + cidx] * carry;
       }
     }
   }
@@ -270,24 +280,21 @@ __device__ void affine_scan(const double *cM, const double *cE, double *cIn, int
 // ---- spot sweep ---------------------------------------------------------------
 // One 128-byte row of per-node spot coefficients, built from the spot table
 // with the boundary handling folded in:
-// [psub' pmid psup' qsub' | qmid qsup' pr qr | S*wsub S*wmid S*wsup snz | S 0 0 0]
+// [psub' pmid psup' qsub' | qmid qsup' pr qr | S*wsub S*wmid S*wsup S^2 | S snz 0 0]
 // (p*: first-derivative bands scaled by r S, with -r/2 on the diagonal; q*:
 // raw second-derivative weights, and qr the raw residual, to be scaled by
 // c00 = (1/2 v) S^2 in the kernel) where ' marks band entries zeroed when
-// they multiply a Dirichlet neighbour
-// (row 0 for row 1; row nS-1 for row nS-2 under a barrier) -- the explicit
-// apply forces those neighbours to 0, and in the solve the coupling to a
-// Dirichlet row (whose value is 0) contributes nothing -- and snz = [S != 0].
-// Dirichlet rows have all-zero bands.
+// they multiply a Dirichlet neighbour (row 0 for row 1; row nS-1 for row nS-2
+// under a barrier) -- the explicit apply forces those neighbours to 0, and in
+// the solve the coupling to a Dirichlet row (whose value is 0) contributes
+// nothing -- and snz = [S != 0].  Dirichlet rows have all-zero bands.  A sweep
+// reads the first twelve fields of a row; S and snz only at row 0 (the spot
+// mesh increases strictly from S >= 0, so only row 0 can have S = 0).
 constexpr int kSpRow = 16;
+constexpr int kSpFields = 12;
 
 struct SpotRow {
-  double psub, pmid, psup, qsub, qmid, qsup, pr, qr, wsub, wmid, wsup, snz, S;
-};
-
-// 3x3 neighbourhood of a field around (i, j): rows i-1, i, i+1; cols j-1, j, j+1.
-struct Win {
-  double v[3][3];
+  double psub, pmid, psup, qsub, qmid, qsup, pr, qr, wsub, wmid, wsup, S2;
 };
 
 // A1 on one variance column scaled by a step weight c (c = dt for the
@@ -300,7 +307,8 @@ struct ColOp {
 };
 
 struct VarCol {  // variance coefficients of column j (constant along a spot line)
-  double wv0, wv1, wv2, v;
+  double wv0, wv1, wv2;
+  double c0h;     // v / 2 (c00 = c0h S^2)
   double rsv;     // rho * sigma * v
   ColOp f1, t1;   // dt A1 and theta dt A1 on this column
 };
@@ -310,8 +318,10 @@ struct Bands {
   double sub, mid, sup, r0;
 };
 
-__device__ __forceinline__ Bands a0_bands(const SpotRow &R, double v) {
-  const double c00 = c00_of(v, R.S);
+// A0 band entry p + w2 * c00 with c00 = (1/2 v) S^2, one rounding each.
+__device__ __forceinline__ Bands a0_bands(const SpotRow &R, double c0h) {
+#pragma clang fp contract(off)
+  const double c00 = c0h * R.S2;
   Bands b;
   b.sub = band_add(R.psub, R.qsub, c00);
   b.mid = band_add(R.pmid, R.qmid, c00);
@@ -320,36 +330,133 @@ __device__ __forceinline__ Bands a0_bands(const SpotRow &R, double v) {
   return b;
 }
 
-// (c A0 f)(i,j) from the window: the entries scaled first, fl(c * a), as the
-// reference's operator c A0 holds them, then summed in its band order (super,
-// main, sub).  The association matters where entries are large: the
-// reference scales the operator and applies it, so applying A0 and scaling
-// the result is a different rounding, which adds up step after step.
-__device__ __forceinline__ double a0s(double c, const Bands &B, const Win &w) {
-  const double sup = c * B.sup, mid = c * B.mid, sub = c * B.sub;
-  return (sup * w.v[2][1] + mid * w.v[1][1]) + sub * w.v[0][1];
-}
-
 __device__ __forceinline__ ColOp col_op(double c, double gsub, double gmid, double gsup,
                                         double gz, double split) {
   return ColOp{c * gsub, c * gmid, c * gsup, c * gz, c * split};
 }
 
-// (c A1 f)(i,j) with the scaled entries K of this column; fjm2 = f(i, j-2)
-// (K.z is zero except on the last column).  S = 0 lines keep only the -r/2
-// of the split.
-__device__ __forceinline__ double a1s(const ColOp &K, double snz, const Win &w, double fjm2) {
-  const double full = ((K.sup * w.v[1][2] + K.mid * w.v[1][1]) + K.sub * w.v[1][0]) + K.z * fjm2;
-  return snz != 0.0 ? full : K.s0 * w.v[1][1];
+// A state row at this lane's column j: columns j-1, j, j+1 (clamped to the
+// grid) and j-2 on the grid's last column (0 elsewhere).
+struct Row4 {
+  double l, c, r, m2;
+};
+
+struct LaneCols {  // this lane's column indices within a row (clamped)
+  unsigned jm, jc, jp, jm2, jx;
+  bool last;
+};
+
+__device__ __forceinline__ Row4 load_row4(const double *__restrict__ f, unsigned rowoff,
+                                          const LaneCols &lc) {
+  const double *p = f + rowoff;
+  Row4 w;
+  w.l = p[lc.jm];
+  w.c = p[lc.jc];
+  w.r = p[lc.jp];
+  w.m2 = lc.last ? p[lc.jm2] : 0.0;
+  return w;
 }
 
-// (A01 f)(i,j) = rho sigma S v (ws x wv) . f, zero on boundary rows and columns
-// (their weights are zero).
-__device__ __forceinline__ double a01w(const SpotRow &R, const VarCol &q, const Win &w) {
-  const double r0 = q.wv0 * w.v[0][0] + q.wv1 * w.v[0][1] + q.wv2 * w.v[0][2];
-  const double r1 = q.wv0 * w.v[1][0] + q.wv1 * w.v[1][1] + q.wv2 * w.v[1][2];
-  const double r2 = q.wv0 * w.v[2][0] + q.wv1 * w.v[2][1] + q.wv2 * w.v[2][2];
-  return q.rsv * (R.wsub * r0 + R.wmid * r1 + R.wsup * r2);
+// A row as loaded for the window: column j and this lane's edge column jx
+// (j - 1 on the tile's first lane, j + 1 on its last, j elsewhere).  The
+// inner neighbours come from the adjacent lanes, which hold exactly those
+// columns (a lane past the grid's last column loads that column, the clamped
+// j + 1 of the last one); column j - 2 of the grid's last column comes from
+// the lane two before it (the tile list puts the last column on a tile's last
+// lane).
+struct Raw2 {
+  double e, c;
+};
+
+__device__ __forceinline__ Raw2 load_raw2(const double *__restrict__ f, unsigned rowoff,
+                                          const LaneCols &lc) {
+  const double *p = f + rowoff;
+  return Raw2{p[lc.jx], p[lc.jc]};
+}
+
+// Value of the lane before / after / two before within the tile's lanes
+// (quad_perm for four columns; row_shr / row_shl within a DPP row of 16
+// lanes for 8 and 16; the tile's first and last lanes use their edge load).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int NL>
+__device__ __forceinline__ double from_prev_col(double x) {
+  return dpp_mov<NL == 4 ? 0x90 : 0x111>(x);  // quad_perm [0,0,1,2] / row_shr:1
+}
+template <int NL>
+__device__ __forceinline__ double from_next_col(double x) {
+  return dpp_mov<NL == 4 ? 0xF9 : 0x101>(x);  // quad_perm [1,2,3,3] / row_shl:1
+}
+template <int NL>
+__device__ __forceinline__ double from_prev2_col(double x) {
+  return dpp_mov<NL == 4 ? 0x40 : 0x112>(x);  // quad_perm [0,0,0,1] / row_shr:2
+}
+
+template <int NL>
+__device__ __forceinline__ Row4 expand_raw(const Raw2 &w, int jj, bool last) {
+  const double l = from_prev_col<NL>(w.c), r = from_next_col<NL>(w.c);
+  const double m2 = from_prev2_col<NL>(w.c);
+  return Row4{jj == 0 ? w.e : l, w.c, jj == NL - 1 ? w.e : r, last ? m2 : 0.0};
+}
+template <int NL>
+__device__ __forceinline__ Row4 expand_raw(const Row4 &w, int, bool) {
+  return w;
+}
+
+// Rows in flight: the edge-and-centre form (neighbours by DPP) or all three
+// columns loaded.
+template <bool NB_DPP>
+struct RawRow {
+  using type = Raw2;
+  static __device__ __forceinline__ Raw2 load(const double *__restrict__ f, unsigned rowoff,
+                                              const LaneCols &lc) {
+    return load_raw2(f, rowoff, lc);
+  }
+  static __device__ __forceinline__ Raw2 zero() { return Raw2{0.0, 0.0}; }
+};
+template <>
+struct RawRow<false> {
+  using type = Row4;
+  static __device__ __forceinline__ Row4 load(const double *__restrict__ f, unsigned rowoff,
+                                              const LaneCols &lc) {
+    return load_row4(f, rowoff, lc);
+  }
+  static __device__ __forceinline__ Row4 zero() { return Row4{0.0, 0.0, 0.0, 0.0}; }
+};
+
+// What the sliding window keeps of a row once it has entered: the value at
+// column j and the row's share of the cross product, wv . (x_{j-1}, x_j,
+// x_{j+1}), of U and of D = Y - U, and Y at column j.  The share is formed
+// once per row and used by the three windows that hold the row.
+struct WinRow {
+  double uc, urs, dc, drs, yc;
+};
+
+template <int MODE>
+__device__ __forceinline__ void enter_row(const VarCol &q, const Row4 &u, const Row4 &y,
+                                          WinRow &w, Row4 &d) {
+  constexpr bool HAS_Y = MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS;
+  w.uc = u.c;
+  w.urs = q.wv0 * u.l + q.wv1 * u.c + q.wv2 * u.r;
+  if constexpr (HAS_Y) {
+    d.l = y.l - u.l;
+    d.c = y.c - u.c;
+    d.r = y.r - u.r;
+    d.m2 = y.m2 - u.m2;
+    w.dc = d.c;
+    w.drs = q.wv0 * d.l + q.wv1 * d.c + q.wv2 * d.r;
+    w.yc = y.c;
+  }
+}
+
+// (c A1 f)(i,j) with the scaled entries K of this column over row i of f (K.z
+// is zero except on the last column).
+__device__ __forceinline__ double a1s(const ColOp &K, const Row4 &w) {
+  return ((K.sup * w.r + K.mid * w.c) + K.sub * w.l) + K.z * w.m2;
 }
 
 // Work-group index -> tile index so that consecutive tiles (which share cache
@@ -368,10 +475,9 @@ __device__ __forceinline__ double quad_bcast(double x) {
   return __hiloint2double(hi, lo);
 }
 
-// Spot-table row shared by a quad of lanes: lane jj of the quad loads doubles
-// [4jj, 4jj+4) of the 128-byte row (two 16-byte loads), so a quad reads each
-// row once, and the fields are broadcast.  Every lane of the quad must
-// execute quad_row_fields.
+// Four-column tiles: the quad's lanes share a table row; lane jj of the quad
+// loads doubles [4jj, 4jj+4) (two 16-byte loads) and the fields are broadcast
+// (every lane of the quad must execute quad_row_fields).
 __device__ __forceinline__ void quad_row_load(const double *__restrict__ spr, int jj, double2 &a,
                                               double2 &b) {
   const double2 *p = reinterpret_cast<const double2 *>(spr + 4 * jj);
@@ -386,133 +492,74 @@ __device__ __forceinline__ SpotRow quad_row_fields(double2 a, double2 b) {
   r.qmid = quad_bcast<1>(a.x); r.qsup = quad_bcast<1>(a.y);
   r.pr = quad_bcast<1>(b.x);   r.qr = quad_bcast<1>(b.y);
   r.wsub = quad_bcast<2>(a.x); r.wmid = quad_bcast<2>(a.y);
-  r.wsup = quad_bcast<2>(b.x); r.snz = quad_bcast<2>(b.y);
-  r.S = quad_bcast<3>(a.x);
+  r.wsup = quad_bcast<2>(b.x); r.S2 = quad_bcast<2>(b.y);
   return r;
 }
 
-// Broadcast lane Q of each row of 16 lanes (DPP row_newbcast:Q).
-template <int Q>
-__device__ __forceinline__ double row16_bcast(double x) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x150 + Q, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x150 + Q, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
+// Wider tiles (8 or 16 columns): the work-group stages the table rows its
+// chunks read into LDS, chunk by chunk (MAXL rows of 12 fields, 2 doubles of
+// padding per chunk so that the chunks of a wave start on different banks),
+// and each lane reads its row with six 16-byte LDS loads (the lanes of a
+// chunk read the same address).
+__device__ __forceinline__ SpotRow lds_row_fields(const double *__restrict__ p) {
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+  const double2 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], f = q[5];
+  return SpotRow{a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y, e.x, e.y, f.x, f.y};
 }
 
-// The same row shared by the 16 lanes of a DPP row (16-column tiles): lane jj
-// loads 16-byte piece jj % 8 of the 128-byte row (lanes 8-15 repeat lanes
-// 0-7), and field f is broadcast from lane f / 2.
-__device__ __forceinline__ SpotRow row16_fields(double2 a) {
-  SpotRow r;
-  r.psub = row16_bcast<0>(a.x); r.pmid = row16_bcast<0>(a.y);
-  r.psup = row16_bcast<1>(a.x); r.qsub = row16_bcast<1>(a.y);
-  r.qmid = row16_bcast<2>(a.x); r.qsup = row16_bcast<2>(a.y);
-  r.pr = row16_bcast<3>(a.x);   r.qr = row16_bcast<3>(a.y);
-  r.wsub = row16_bcast<4>(a.x); r.wmid = row16_bcast<4>(a.y);
-  r.wsup = row16_bcast<5>(a.x); r.snz = row16_bcast<5>(a.y);
-  r.S = row16_bcast<6>(a.x);
-  return r;
-}
-
-// Table-row share of lane jj for tiles of NL columns (see above).
-template <int NL>
-__device__ __forceinline__ void share_row_load(const double *__restrict__ spr, int jj,
-                                               double2 &a, double2 &b) {
-  if constexpr (NL == 4) {
-    quad_row_load(spr, jj, a, b);
-  } else {
-    a = reinterpret_cast<const double2 *>(spr)[jj & 7];
-    b = a;
-  }
-}
-
-template <int NL>
-__device__ __forceinline__ SpotRow share_row_fields(double2 a, double2 b) {
-  if constexpr (NL == 4)
-    return quad_row_fields(a, b);
-  else
-    return row16_fields(a);
-}
-
-// Row rr (already clamped to the grid) of a field at columns jm, j, jp, and
-// at column jm2 = j - 2 on the last column only (the second-order backward
-// row of d/dv reaches two columns back; elsewhere out[3] = 0).  Offsets are
-// 32-bit unsigned (an option's grid has at most 2048 x 1024 nodes).
-// Neighbouring columns of a tile: the value of lane jj - 1 / jj + 1 of the
-// quad (NL = 4, DPP quad_perm [0,0,1,2] / [1,2,3,3]) or of the DPP row
-// (NL = 16, row_shr:1 / row_shl:1).  Lanes 0 and NL - 1 get no neighbour
-// from these (they use their own edge load).
-template <int NL>
-__device__ __forceinline__ double from_prev_col(double x) {
-  constexpr int ctrl = NL == 4 ? 0x90 : 0x111;
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), ctrl, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-template <int NL>
-__device__ __forceinline__ double from_next_col(double x) {
-  constexpr int ctrl = NL == 4 ? 0xF9 : 0x101;
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), ctrl, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
-// One state row as loaded: [0] the edge column jx (j - 1 on the tile's first
-// lane, j + 1 on its last, j elsewhere), [1] column j, [3] column j - 2 on
-// the grid's last column.  expand_row turns it into the window row
-// [j - 1, j, j + 1, j - 2] by taking the inner neighbours from the adjacent
-// lanes, which hold exactly those columns (a lane past the grid's last column
-// loads that column, which is also the clamped j + 1 of the last one).
-__device__ __forceinline__ void load_row_raw(const double *__restrict__ f, unsigned rr,
-                                             unsigned nv, unsigned jx, unsigned j, unsigned jm2,
-                                             bool last, double out[4]) {
-  const unsigned b = rr * nv;
-  out[0] = f[b + jx];
-  out[1] = f[b + j];
-  out[2] = 0.0;
-  out[3] = last ? f[b + jm2] : 0.0;
-}
-template <int NL>
-__device__ __forceinline__ void expand_row(int jj, double w[4]) {
-  const double l = from_prev_col<NL>(w[1]);
-  const double r = from_next_col<NL>(w[1]);
-  w[2] = jj == NL - 1 ? w[0] : r;
-  w[0] = jj == 0 ? w[0] : l;
-}
+// An empty asm that reads x: x has to be computed before this point.  The
+// row walk keeps its results in registers (no store orders it), so without
+// these fences the compiler hoists every row's loads to the top of the
+// unrolled walk and runs out of registers.
+__device__ __forceinline__ void pin(double x) { asm volatile("" ::"v"(x)); }
 
 // A spot sweep: one work-group per (option, NL adjacent variance columns,
-// NL = 4 or 16); thread (jj, ch) = (tid % NL, tid / NL) owns rows
-// [ch*L, ch*L+L) of column j0+jj through every phase.  The NL lanes of a quad
-// (NL = 4) or of a DPP row (NL = 16) work on the same rows, so they share the
-// spot-table rows.  Four columns per tile keep a large grid spread over every
-// CU (2048 x 1024: 256 tiles); sixteen make each load of a wave touch 4 rows
-// of 128 bytes instead of 16 rows of 32 bytes, which pays where the batch has
-// tiles to spare.  The row walk keeps its loads
-// two rows ahead of the arithmetic (table rows, pivots, and the state rows
-// entering the 3x3 windows); a scheduling barrier per row stops the compiler
-// from hoisting more than that, which would exhaust the registers.  Loads use
-// clamped indices; rows past the chunk end are computed and discarded.
+// NL = 4, 8 or 16); thread (jj, ch) = (tid % NL, tid / NL) owns rows
+// [ch*L, ch*L+L) of column j0+jj through every phase.  The NL lanes of a
+// chunk work on the same rows, so they share the spot-table rows.  Four
+// columns per tile keep a large grid spread over every CU (2048 x 1024: 256
+// tiles); wider tiles make each load of a wave touch fewer, longer row
+// segments, which pays where the batch has tiles to spare.
+//
+// The row walk keeps a window of three rows (i-1, i, i+1) of what the stage
+// needs from U and Y2, the loads of rows i+2 and i+3 in flight, and forms the
+// fused right-hand side (and the next stage's explicit term) of row i.  The
+// state rows are loaded with clamped indices; rows past the chunk are
+// computed and discarded.
 //
 // The LU solve is run in scaled form: with pivots rd = 1/u, multipliers
 // mu_i = -a_i rd_i and n_i = -c_i rd_i, the forward sweep computes
 // y'_i = rd_i d_i + mu_i y'_{i-1} (= rd_i y_i) and the backward sweep
 // x_i = y'_i + n_i x_{i+1}, so the pivots are needed only while the
-// right-hand side is formed.  Registers: mu and n of the chunk.  LDS, in
-// thread-private slots [k*THREADS + tid]: y' (then x) and the next stage's
-// explicit term; plus the 3*TPL*NL chunk carries of the scans.
+// right-hand side is formed.  The pivots rd_i = 1 / (b_i - (a_i rd_{i-1})
+// c_{i-1}) are recomputed down the chunk from the stored pivot of the row
+// before it (the recurrence the factorisation ran).  mu, n and y' (then x)
+// stay in registers; LDS holds the next stage's explicit term (thread-private
+// slots [k*THREADS + tid]), the chunk carries of the scans and, for wide
+// tiles, the table rows.
+//
+// Rows k >= L of a chunk belong to the next chunk: the uniform guard k < L
+// keeps them out of both chunk maps.  In the last chunk, rows past the grid
+// (i >= nS) have finite values and follow row nS - 1, whose super-diagonal is
+// zero (n = 0), so they do not reach any valid row either way.
 template <int MODE, int THREADS, int NL, int MAXL>
 __global__ void __launch_bounds__(THREADS)
 spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ U,
                   const double *__restrict__ Y, double *__restrict__ out) {
-  static_assert(NL == 4 || NL == 16, "spot tiles are a quad or a DPP row of columns wide");
+  static_assert(NL == 4 || NL == 8 || NL == 16, "spot tiles are 4, 8 or 16 columns wide");
   extern __shared__ double lds[];
   constexpr int TPL = THREADS / NL;
   constexpr bool HAS_Y = MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS;
-  double *db = lds;
-  double *eb = db + MAXL * THREADS;
-  double *cM = eb + MAXL * THREADS;
-  double *cE = cM + TPL * NL;
-  double *cIn = cE + TPL * NL;
+  constexpr bool HAS_EX = MODE != SM_IMPL;
+  constexpr bool TAB_LDS = NL != 4;
+  constexpr int CHP = MAXL * kSpFields + 2;  // LDS table: doubles per chunk
+  constexpr int PF = 4;                      // state rows in flight
+  constexpr bool NB_DPP = true;              // neighbour columns by DPP
+  double *cM = lds;
+  double *cE = cM + NL * (TPL + 1);
+  double *cIn = cE + NL * (TPL + 1);
+  double *eb = cIn + NL * (TPL + 1);  // next stage's explicit term, slot [k * THREADS + tid]
+  double *tab = eb + (HAS_EX ? MAXL * THREADS : 0);  // 16-byte aligned
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const HOpt o = c.opts[t.opt];
   const int nS = (int)o.nS, nv = (int)o.nv, j0 = t.start;
@@ -523,8 +570,6 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   const int jc = col_ok ? j : nv - 1;  // clamped column for loads
   const int L = (nS + TPL - 1) / TPL;
   const int s = ch * L;
-  const int e = s + L < nS ? s + L : nS;
-  const bool active = col_ok && s < e;
   const double *u = U + o.off;
   const double *y2 = HAS_Y ? Y + o.off : nullptr;
   const double *rd = c.rd0 + o.off;
@@ -532,193 +577,277 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   const unsigned unv = (unsigned)nv;
   auto clamp_row = [&](int i) { return (unsigned)(i < 0 ? 0 : (i >= nS ? nS - 1 : i)); };
 
+  if constexpr (TAB_LDS) {
+    // table rows of every chunk: TPL * MAXL slots (rows past the grid zero)
+#pragma unroll
+    for (int slot = tid; slot < TPL * MAXL; slot += THREADS) {
+      const int cc = slot / MAXL, k = slot % MAXL;
+      const int r = cc * L + k;
+      const double2 *src = reinterpret_cast<const double2 *>(spr + (int64_t)(r < nS ? r : 0) * kSpRow);
+      double2 *dst = reinterpret_cast<double2 *>(tab + cc * CHP + k * kSpFields);
+      const bool in = r < nS;
+#pragma unroll
+      for (int part = 0; part < 6; ++part) {
+        const double2 v = src[part];
+        dst[part] = in ? v : make_double2(0.0, 0.0);
+      }
+    }
+  }
+
   VarCol q;
-  q.wv0 = VT(c, o, VT_WSUB, jc); q.wv1 = VT(c, o, VT_WMID, jc); q.wv2 = VT(c, o, VT_WSUP, jc);
-  q.v = VT(c, o, VT_V, jc);
-  q.rsv = (o.rho * o.sigma) * q.v;
   const bool last = jc == nv - 1;
   {
+    const double v = VT(c, o, VT_V, jc);
+    q.wv0 = VT(c, o, VT_WSUB, jc); q.wv1 = VT(c, o, VT_WMID, jc); q.wv2 = VT(c, o, VT_WSUP, jc);
+    q.c0h = 0.5 * v;
+    q.rsv = (o.rho * o.sigma) * v;
     const double gsub = VT(c, o, VT_GSUB, jc), gmid = VT(c, o, VT_GMID, jc),
                  gsup = VT(c, o, VT_GSUP, jc), gz = last ? o.Gz : 0.0, split = -o.r / 2.0;
     q.f1 = col_op(o.dt, gsub, gmid, gsup, gz, split);
     q.t1 = col_op(o.tdt, gsub, gmid, gsup, gz, split);
   }
-  const unsigned ujc = (unsigned)jc;
-  const unsigned jm = jc > 0 ? ujc - 1 : 0u, jp = jc < nv - 1 ? ujc + 1 : ujc;
-  const unsigned jm2 = jc > 1 ? ujc - 2 : 0u;
-  const unsigned jx = jj == 0 ? jm : (jj == NL - 1 ? jp : ujc);  // this lane's edge load
+  LaneCols lc;
+  lc.jc = (unsigned)jc;
+  lc.jm = jc > 0 ? lc.jc - 1 : 0u;
+  lc.jp = jc < nv - 1 ? lc.jc + 1 : lc.jc;
+  lc.jm2 = jc > 1 ? lc.jc - 2 : 0u;
+  lc.jx = jj == 0 ? lc.jm : (jj == NL - 1 ? lc.jp : lc.jc);
+  lc.last = last;
   const double tdt = o.tdt, dt = o.dt;
-  const int dir_hi = o.barrier ? nS - 1 : -1;  // Dirichlet rows: 0 and dir_hi
+  const bool barrier = o.barrier != 0;
+  // S = 0 line (row 0 when the mesh starts at 0): A1 keeps only the -r/2 split
+  const bool s0_zero = s == 0 && spr[13] == 0.0;
 
-  // Phase 1: walk the chunk with 3x3 windows, form the fused right-hand side
-  // (and the next stage's explicit term), and run the local forward recurrence.
-  double mu[MAXL], nreg[MAXL];
-  // rows i-1, i, i+1 (expanded windows) and i+2 (in flight, as loaded);
-  // [3] = column j-2 on the last column
-  double wu[4][4], wy[4][4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-    load_row_raw(u, clamp_row(s - 1 + r), unv, jx, ujc, jm2, last, wu[r]);
-  if (HAS_Y) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      load_row_raw(y2, clamp_row(s - 1 + r), unv, jx, ujc, jm2, last, wy[r]);
-  }
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    expand_row<NL>(jj, wu[r]);
-    if (HAS_Y) expand_row<NL>(jj, wy[r]);
-  }
-  double2 ta0, tb0, ta1, tb1;  // table rows i and i+1
-  share_row_load<NL>(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
-  share_row_load<NL>(spr + clamp_row(s + 1) * kSpRow, jj, ta1, tb1);
-  // Pivots: rd_i = 1 / (b_i - (a_i rd_{i-1}) c_{i-1}) down the chunk, from the
-  // stored pivot and super-diagonal of the row before the chunk (the same
-  // recurrence the factorisation ran).
+  // Pivot recurrence state from the row before the chunk.
   double rdp = 0.0, cp = 0.0;
-  {
-    double2 tpa, tpb;
-    share_row_load<NL>(spr + clamp_row(s - 1) * kSpRow, jj, tpa, tpb);
-    const SpotRow Rp = share_row_fields<NL>(tpa, tpb);
-    if (s > 0) {
-      rdp = rd[clamp_row(s - 1) * unv + ujc];
-      cp = -tdt * a0_bands(Rp, q.v).sup;
-    }
+  if (s > 0) {
+    const double *pr = spr + (int64_t)(s - 1) * kSpRow;
+    const double c00 = q.c0h * pr[11];
+    rdp = rd[(unsigned)(s - 1) * unv + lc.jc];
+    cp = -tdt * band_add(pr[2], pr[5], c00);
   }
-  double M = 1.0, E = 0.0;
+
+  // Window: rows s-1 (a) and s (b) entered; rows s+1 .. s+PF in flight
+  // (row s+1+m in ring slot m % PF).
+  WinRow wa, wb, wc;
+  Row4 ub, db, yb, uc4, dc4, yc4;
+  {
+    const Row4 z{0.0, 0.0, 0.0, 0.0};
+    Row4 ua = load_row4(u, clamp_row(s - 1) * unv, lc), ya = z, da;
+    ub = load_row4(u, clamp_row(s) * unv, lc);
+    yb = z;
+    if constexpr (HAS_Y) {
+      ya = load_row4(y2, clamp_row(s - 1) * unv, lc);
+      yb = load_row4(y2, clamp_row(s) * unv, lc);
+    }
+    enter_row<MODE>(q, ua, ya, wa, da);
+    enter_row<MODE>(q, ub, yb, wb, db);
+  }
+  using RR = RawRow<NB_DPP>;
+  typename RR::type ru[PF], ry[PF];
+#pragma unroll
+  for (int m = 0; m < PF; ++m) {
+    const unsigned rr = clamp_row(s + 1 + m) * unv;
+    ru[m] = RR::load(u, rr, lc);
+    ry[m] = RR::zero();
+    if constexpr (HAS_Y) ry[m] = RR::load(y2, rr, lc);
+  }
+  // four-column tiles: table rows i and i+1 in flight
+  double2 ta0, tb0, ta1, tb1;
+  if constexpr (!TAB_LDS) {
+    quad_row_load(spr + clamp_row(s) * kSpRow, jj, ta0, tb0);
+    quad_row_load(spr + clamp_row(s + 1) * kSpRow, jj, ta1, tb1);
+  }
+  if constexpr (TAB_LDS) __syncthreads();
+  const double *tch = tab + ch * CHP;
+
+  double mu[MAXL], nreg[MAXL], dv[MAXL];
 #pragma unroll
   for (int k = 0; k < MAXL; ++k) {
     const int i = s + k;
-    // loads for two rows ahead
+    const int slot = k % PF;
+    // the table row of row i
+    SpotRow R;
     double2 ta2, tb2;
-    const unsigned i2 = clamp_row(i + 2), i3 = clamp_row(i + 3);
-    share_row_load<NL>(spr + i2 * kSpRow, jj, ta2, tb2);
-    double nu[4], ny[4];
-    load_row_raw(u, i3, unv, jx, ujc, jm2, last, nu);
-    if (HAS_Y) load_row_raw(y2, i3, unv, jx, ujc, jm2, last, ny);
-    // row i+1 enters the window
-    expand_row<NL>(jj, wu[2]);
-    if (HAS_Y) expand_row<NL>(jj, wy[2]);
-    const double uj2 = wu[1][3];
-    const double yj2 = HAS_Y ? wy[1][3] : 0.0;
-
-    const SpotRow R = share_row_fields<NL>(ta0, tb0);
-    Win wU, wY;
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        wU.v[r][b] = wu[r][b];
-        wY.v[r][b] = HAS_Y ? wy[r][b] : 0.0;
-      }
-    double rhs = 0.0, ex = 0.0;
-    const Bands B = a0_bands(R, q.v);
-    if (MODE == SM_IMPL) {
-      rhs = wU.v[1][1] + dt * B.r0;
+    if constexpr (TAB_LDS) {
+      R = lds_row_fields(tch + k * kSpFields);
     } else {
+      quad_row_load(spr + clamp_row(i + 2) * kSpRow, jj, ta2, tb2);
+    }
+    // row i + 1 enters the window; its slot takes the loads of row i + 1 + PF
+    {
+      const Row4 un = expand_raw<NL>(ru[slot], jj, last);
+      Row4 yn{0.0, 0.0, 0.0, 0.0};
+      if constexpr (HAS_Y) yn = expand_raw<NL>(ry[slot], jj, last);
+      enter_row<MODE>(q, un, yn, wc, dc4);
+      uc4 = un;
+      yc4 = yn;
+      const unsigned rr = clamp_row(i + 1 + PF) * unv;
+      ru[slot] = RR::load(u, rr, lc);
+      if constexpr (HAS_Y) ry[slot] = RR::load(y2, rr, lc);
+    }
+    if constexpr (!TAB_LDS) R = quad_row_fields(ta0, tb0);
+
+    const Bands B = a0_bands(R, q.c0h);
+    // the entries of dt A0 and theta dt A0, fl(c * a), as the reference's
+    // scaled operators hold them; applied in its band order (super, main, sub)
+    const double tsub = tdt * B.sub, tmid = tdt * B.mid, tsup = tdt * B.sup;
+    double rhs = 0.0, ex = 0.0;
+    if constexpr (MODE == SM_IMPL) {
+      rhs = wb.uc + dt * B.r0;
+    } else {
+      const double dsub = dt * B.sub, dmid = dt * B.mid, dsup = dt * B.sup;
+      auto a0d = [&](double xa, double xb, double xc) {
+        return (dsup * xc + dmid * xb) + dsub * xa;
+      };
+      auto a0t = [&](double xa, double xb, double xc) {
+        return (tsup * xc + tmid * xb) + tsub * xa;
+      };
+      // (A01 f)(i, j) from the rows' shares
+      auto x01 = [&](double ra, double rb, double rc) {
+        return q.rsv * (R.wsub * ra + R.wmid * rb + R.wsup * rc);
+      };
+      // S = 0 line: A1 keeps only -r/2 (possible at row 0 only)
+      const bool zline = k == 0 && s0_zero;
+      auto a1 = [&](const ColOp &K, const Row4 &w) {
+        const double full = a1s(K, w);
+        return zline ? K.s0 * w.c : full;
+      };
       // Y0 = U + dt A0 U + dt R0 + dt A01 U + dt A1 U, in the reference's
-      // order of its explicit operators (spot, cross, variance) with each
-      // operator's entries scaled by dt
-      const double y0 = ((wU.v[1][1] + (a0s(dt, B, wU) + dt * B.r0)) + dt * a01w(R, q, wU)) +
-                        a1s(q.f1, R.snz, wU, uj2);
-      if (MODE == SM_FIRST) {
-        rhs = y0 - a0s(tdt, B, wU);
-        ex = a1s(q.t1, R.snz, wU, uj2);
-      } else {
+      // order of its explicit operators (spot, cross, variance)
+      const double y0 = ((wb.uc + (a0d(wa.uc, wb.uc, wc.uc) + dt * B.r0)) +
+                         dt * x01(wa.urs, wb.urs, wc.urs)) +
+                        a1(q.f1, ub);
+      if constexpr (MODE == SM_FIRST) {
+        rhs = y0 - a0t(wa.uc, wb.uc, wc.uc);
+        ex = a1(q.t1, ub);
+      } else if constexpr (MODE == SM_HV) {
         // the second-stage corrections act on the difference of the states
         // (FiniteDifferenceEngine.pyx:744-765, 818-835, 881-894)
-        Win wD;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int b = 0; b < 3; ++b) wD.v[r][b] = wY.v[r][b] - wU.v[r][b];  // Y2 - U
-        const double dj2 = yj2 - uj2;
-        if (MODE == SM_HV) {
-          // V = Y0 - 1/2 F (U - Y2) over the explicit operators (no residual)
-          const double v1 = ((y0 + 0.5 * a0s(dt, B, wD)) + 0.5 * (dt * a01w(R, q, wD))) +
-                            0.5 * a1s(q.f1, R.snz, wD, dj2);
-          rhs = v1 - a0s(tdt, B, wY);
-          ex = a1s(q.t1, R.snz, wY, yj2);
-        } else if (MODE == SM_CS) {
-          rhs = (y0 + (0.5 * dt) * a01w(R, q, wD)) - a0s(tdt, B, wU);
-          ex = a1s(q.t1, R.snz, wU, uj2);
-        } else {  // SM_MCS: (1/2 - theta) F (Y2 - U), residual included
-          const double h = 0.5 - o.th;
-          const double v1 = (((y0 + tdt * a01w(R, q, wD)) + h * (a0s(dt, B, wD) + dt * B.r0)) +
-                             h * (dt * a01w(R, q, wD))) +
-                            h * a1s(q.f1, R.snz, wD, dj2);
-          rhs = v1 - a0s(tdt, B, wU);
-          ex = a1s(q.t1, R.snz, wU, uj2);
-        }
+        // V = Y0 - 1/2 F (U - Y2) over the explicit operators (no residual)
+        const double v1 = ((y0 + 0.5 * a0d(wa.dc, wb.dc, wc.dc)) +
+                           0.5 * (dt * x01(wa.drs, wb.drs, wc.drs))) +
+                          0.5 * a1(q.f1, db);
+        rhs = v1 - a0t(wa.yc, wb.yc, wc.yc);
+        ex = a1(q.t1, yb);
+      } else if constexpr (MODE == SM_CS) {
+        rhs = (y0 + (0.5 * dt) * x01(wa.drs, wb.drs, wc.drs)) - a0t(wa.uc, wb.uc, wc.uc);
+        ex = a1(q.t1, ub);
+      } else {  // SM_MCS: (1/2 - theta) F (Y2 - U), residual included
+        const double h = 0.5 - o.th;
+        const double xd = x01(wa.drs, wb.drs, wc.drs);
+        const double v1 = (((y0 + tdt * xd) + h * (a0d(wa.dc, wb.dc, wc.dc) + dt * B.r0)) +
+                           h * (dt * xd)) +
+                          h * a1(q.f1, db);
+        rhs = v1 - a0t(wa.uc, wb.uc, wc.uc);
+        ex = a1(q.t1, ub);
       }
     }
-    if (i == 0 || i == dir_hi) rhs = 0.0;  // the solve forces Dirichlet rows to 0
+    // the solve forces Dirichlet rows (0, and nS-1 under a barrier) to 0
+    if ((k == 0 && s == 0) || (barrier && i == nS - 1)) rhs = 0.0;
     // I - tdt*A0 on this row (identity rows on Dirichlet rows and zero
-    // off-diagonals at the line ends come from the bands), scaled by the pivot.
-    const double aL = -tdt * B.sub;
-    const double bL = 1.0 - tdt * B.mid;
-    const double cL = -tdt * B.sup;
-    const double rd0 = 1.0 / (bL - (aL * rdp) * cp);
+    // off-diagonals at the line ends come from the bands), scaled by the pivot
+    const double rd0 = 1.0 / ((1.0 - tmid) - (-tsub * rdp) * cp);
     rdp = rd0;
-    cp = cL;
+    cp = -tsup;
     const double dsc = rd0 * rhs;
-    mu[k] = -aL * rd0;
-    nreg[k] = -cL * rd0;
-    db[k * THREADS + tid] = dsc;
-    eb[k * THREADS + tid] = ex;
-    if (i < e) {
-      E = dsc + mu[k] * E;
-      M = mu[k] * M;
+    mu[k] = tsub * rd0;
+    nreg[k] = tsup * rd0;
+    dv[k] = dsc;
+    if constexpr (HAS_EX) eb[k * THREADS + tid] = ex;
+    // slide the window
+    wa = wb;
+    wb = wc;
+    ub = uc4;
+    if constexpr (HAS_Y) {
+      db = dc4;
+      yb = yc4;
     }
-    // rotate the rings
-    ta0 = ta1; tb0 = tb1; ta1 = ta2; tb1 = tb2;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      wu[0][b] = wu[1][b]; wu[1][b] = wu[2][b]; wu[2][b] = wu[3][b]; wu[3][b] = nu[b];
-      if (HAS_Y) {
-        wy[0][b] = wy[1][b]; wy[1][b] = wy[2][b]; wy[2][b] = wy[3][b]; wy[3][b] = ny[b];
-      }
+    if constexpr (!TAB_LDS) {
+      ta0 = ta1; tb0 = tb1; ta1 = ta2; tb1 = tb2;
     }
+    // this row's results before the next row's loads (registers)
+    pin(dsc);
+    pin(rdp);
+    if constexpr (HAS_EX) pin(ex);
     __builtin_amdgcn_sched_barrier(0);
   }
-  cM[ch * NL + jj] = active ? M : 1.0;
-  cE[ch * NL + jj] = active ? E : 0.0;
+  // Chunk maps and fix-ups.  Rows k >= L belong to the next chunk; L is
+  // uniform, so full chunks (L == MAXL, the common case) take unguarded loops.
+  const bool full = L == MAXL;
+  const bool active = col_ok && s < nS;
+  double M = 1.0, E = 0.0;
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+      E = dv[k] + mu[k] * E;
+      M = mu[k] * M;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+      if (k < L) {
+        E = dv[k] + mu[k] * E;
+        M = mu[k] * M;
+      }
+    }
+  }
+  cM[jj * (TPL + 1) + ch] = active ? M : 1.0;
+  cE[jj * (TPL + 1) + ch] = active ? E : 0.0;
   __syncthreads();
-  affine_scan<THREADS, NL, false>(cM, cE, cIn, TPL);
+  affine_scan<THREADS, NL, false>(cM, cE, cIn);
   __syncthreads();
   // Forward fix-up, then the local backward recurrence x = y' + n x'.
   {
-    double yv = cIn[ch * NL + jj];
+    double yv = cIn[jj * (TPL + 1) + ch];
 #pragma unroll
     for (int k = 0; k < MAXL; ++k) {
-      yv = db[k * THREADS + tid] + mu[k] * yv;
-      db[k * THREADS + tid] = yv;
+      yv = dv[k] + mu[k] * yv;
+      dv[k] = yv;
     }
   }
   M = 1.0;
   E = 0.0;
+  if (full) {
 #pragma unroll
-  for (int k = MAXL - 1; k >= 0; --k) {
-    if (s + k < e) {
-      E = db[k * THREADS + tid] + nreg[k] * E;
+    for (int k = MAXL - 1; k >= 0; --k) {
+      E = dv[k] + nreg[k] * E;
       M = nreg[k] * M;
+    }
+  } else {
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k) {
+      if (k < L) {
+        E = dv[k] + nreg[k] * E;
+        M = nreg[k] * M;
+      }
     }
   }
   __syncthreads();  // everyone done reading cIn of the forward scan
-  cM[ch * NL + jj] = active ? M : 1.0;
-  cE[ch * NL + jj] = active ? E : 0.0;
+  cM[jj * (TPL + 1) + ch] = active ? M : 1.0;
+  cE[jj * (TPL + 1) + ch] = active ? E : 0.0;
   __syncthreads();
-  affine_scan<THREADS, NL, true>(cM, cE, cIn, TPL);
+  affine_scan<THREADS, NL, true>(cM, cE, cIn);
   __syncthreads();
   // Backward fix-up; stage output x - tdt*A1(f) written straight from the chunk.
   double *ob = out + o.off;
-  double x = cIn[ch * NL + jj];
+  double x = cIn[jj * (TPL + 1) + ch];
+  const unsigned uj = (unsigned)j;
+  if (full) {
 #pragma unroll
-  for (int k = MAXL - 1; k >= 0; --k) {
-    const int i = s + k;
-    if (i < e) {
-      x = db[k * THREADS + tid] + nreg[k] * x;
-      if (col_ok) ob[i * nv + j] = x - eb[k * THREADS + tid];
+    for (int k = MAXL - 1; k >= 0; --k) {
+      const int i = s + k;
+      x = dv[k] + nreg[k] * x;
+      if (col_ok && i < nS) ob[(unsigned)i * unv + uj] = HAS_EX ? x - eb[k * THREADS + tid] : x;
+    }
+  } else {
+#pragma unroll
+    for (int k = MAXL - 1; k >= 0; --k) {
+      const int i = s + k;
+      if (k < L) {
+        x = dv[k] + nreg[k] * x;
+        if (col_ok && i < nS) ob[(unsigned)i * unv + uj] = HAS_EX ? x - eb[k * THREADS + tid] : x;
+      }
     }
   }
 }
@@ -936,7 +1065,7 @@ __global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *ro
   col[SP_S * sp_n] = S;
   col[SP_DIR * sp_n] = dir ? 1.0 : 0.0;
   // AoS row of the spot sweep (layout at kSpRow): neighbour-of-Dirichlet band
-  // entries zeroed, cross weights premultiplied by S, [S != 0] flag.
+  // entries zeroed, cross weights premultiplied by S, S^2, S, [S != 0] flag.
   double *row = spr + g * 16;
   const bool zsub = i == 1;                       // row 0 is Dirichlet
   const bool zsup = o.barrier && i == n - 2;      // row n-1 is Dirichlet
@@ -951,9 +1080,11 @@ __global__ void spot_tables_kernel(const HOpt *opts, int nopt, const int64_t *ro
   row[8] = S * ws[0];
   row[9] = S * ws[1];
   row[10] = S * ws[2];
-  row[11] = S != 0.0 ? 1.0 : 0.0;
+  row[11] = S * S;
   row[12] = S;
-  for (int f = 13; f < 16; ++f) row[f] = 0.0;
+  row[13] = S != 0.0 ? 1.0 : 0.0;
+  row[14] = 0.0;
+  row[15] = 0.0;
 }
 
 // One thread per variance node: A1 rows for S != 0 lines and cross weights.
@@ -1182,7 +1313,7 @@ struct fdadi_hb {
   int var_tpl = 1, var_rows = 256;  // variance sweep: lanes per line, rows per 256-thread tile
   std::vector<int> order;           // options, largest first
   int spot_maxl = 8;  // rows per thread in the spot sweep (8, or 16 for long lines)
-  int spot_nl = 4;    // columns per spot-sweep tile (4 or 16)
+  int spot_nl = 4;    // columns per spot-sweep tile (4, 8 or 16)
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
   bool dev_scalars = false;        // last configure took theta and dt from device memory
@@ -1219,16 +1350,16 @@ Ctx ctx_of(const fdadi_hb *hb) {
   } while (0)
 
 // Spot-sweep shape for lines of length n: `threads` per work-group (0 =
-// default), `nl` columns per tile (4 or 16), and the rows per thread (8, or
-// 16 for long lines).  The register budget allows 16 rows per thread only at
-// <= 512 threads (two waves per SIMD).
+// default), `nl` columns per tile (4, 8 or 16), and the rows per thread (8,
+// or 16 for long lines).  A thread keeps four values per row in registers, so
+// a work-group has at most 512 threads (two waves per SIMD).
 int spot_shape(int64_t n, int threads, int nl, int *t_out, int *maxl_out) {
-  if (nl != 4 && nl != 16) {
-    set_error("fdadi_hb_set_shape: spot tiles are 4 or 16 columns wide");
+  if (nl != 4 && nl != 8 && nl != 16) {
+    set_error("fdadi_hb_set_shape: spot tiles are 4, 8 or 16 columns wide");
     return FDADI_ERR_ARG;
   }
   if (threads == 0) {
-    if (nl == 4 && n <= 1024) {
+    if (nl == 4 && n <= 512) {
       const int64_t need = (n + kMAXLs - 1) / kMAXLs * 4;
       int t = 64;
       while (t < need) t *= 2;
@@ -1239,9 +1370,9 @@ int spot_shape(int64_t n, int threads, int nl, int *t_out, int *maxl_out) {
     threads = 512;
   }
   const int tpl = threads / nl;
-  const bool t_ok = threads >= 64 && threads <= 1024 && (threads & (threads - 1)) == 0 &&
-                    tpl >= 1;
-  if (t_ok && (int64_t)tpl * kMAXLs >= n && !(nl == 16 && threads > 512)) {
+  const bool t_ok = threads >= 64 && threads <= 512 && (threads & (threads - 1)) == 0 &&
+                    tpl >= 1 && (nl == 4 || threads >= 256);
+  if (t_ok && (int64_t)tpl * kMAXLs >= n) {
     *t_out = threads;
     *maxl_out = kMAXLs;
   } else if (t_ok && (int64_t)tpl * 2 * kMAXLs >= n && (threads == 256 || threads == 512)) {
@@ -1259,7 +1390,13 @@ int build_spot_tiles(fdadi_hb *hb) {
   std::vector<Tile> st;
   for (int k : hb->order) {
     const HOpt &o = hb->h_opts[k];
-    for (int64_t j0 = 0; j0 < o.nv; j0 += hb->spot_nl) st.push_back(Tile{k, (int32_t)j0});
+    // the last tile ends on the last column (it may overlap the one before,
+    // whose shared columns both compute identically): the kernel takes
+    // column nv - 3 of the last column from the lane two before it
+    for (int64_t j0 = 0; j0 < o.nv; j0 += hb->spot_nl) {
+      const int64_t start = j0 + hb->spot_nl > o.nv && o.nv >= hb->spot_nl ? o.nv - hb->spot_nl : j0;
+      st.push_back(Tile{k, (int32_t)start});
+    }
   }
   FDADI_REQUIRE((int64_t)st.size() <= hb->cap_stiles, FDADI_ERR_STATE,
                 "spot tile list exceeds its capacity");
@@ -1312,7 +1449,12 @@ template <int MODE, int T, int NL, int ML>
 int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                   hipStream_t s) {
   constexpr int TPL = T / NL;
-  const size_t lds = sizeof(double) * (2 * ML * T + 3 * TPL * NL);
+  // chunk carries of the scans, the next stage's explicit term (MAXL per
+  // thread); wide tiles add their table rows (MAXL rows of 12 fields + 2
+  // doubles of padding per chunk)
+  const size_t ex = MODE == SM_IMPL ? 0 : (size_t)ML * T;
+  const size_t tab = NL == 4 ? 0 : (size_t)TPL * (ML * 12 + 2);
+  const size_t lds = sizeof(double) * (3 * (size_t)NL * (TPL + 1) + ex + tab);
   FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, NL, ML>, lds));
   hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, NL, ML>), dim3(hb->nstiles), dim3(T), lds, s,
                      ctx_of(hb), hb->d_stiles, U, Y, out);
@@ -1320,21 +1462,26 @@ int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *
   return FDADI_OK;
 }
 
+template <int MODE, int NL>
+int launch_spot_wide(const fdadi_hb *hb, const double *U, const double *Y, double *out,
+                     hipStream_t s) {
+  const bool long_rows = hb->spot_maxl == 2 * kMAXLs;
+  switch (hb->spot_threads * 2 + (long_rows ? 1 : 0)) {
+    case 512: return launch_spot_t<MODE, 256, NL, kMAXLs>(hb, U, Y, out, s);
+    case 513: return launch_spot_t<MODE, 256, NL, 2 * kMAXLs>(hb, U, Y, out, s);
+    case 1024: return launch_spot_t<MODE, 512, NL, kMAXLs>(hb, U, Y, out, s);
+    case 1025: return launch_spot_t<MODE, 512, NL, 2 * kMAXLs>(hb, U, Y, out, s);
+  }
+  set_error("spot sweep: unsupported shape for 8- or 16-column tiles");
+  return FDADI_ERR_UNSUPPORTED;
+}
+
 template <int MODE>
 int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                 hipStream_t s) {
-  const bool long_rows = hb->spot_maxl == 2 * kMAXLs;
-  if (hb->spot_nl == 16) {
-    switch (hb->spot_threads * 2 + (long_rows ? 1 : 0)) {
-      case 512: return launch_spot_t<MODE, 256, 16, kMAXLs>(hb, U, Y, out, s);
-      case 513: return launch_spot_t<MODE, 256, 16, 2 * kMAXLs>(hb, U, Y, out, s);
-      case 1024: return launch_spot_t<MODE, 512, 16, kMAXLs>(hb, U, Y, out, s);
-      case 1025: return launch_spot_t<MODE, 512, 16, 2 * kMAXLs>(hb, U, Y, out, s);
-    }
-    set_error("spot sweep: unsupported shape for 16-column tiles");
-    return FDADI_ERR_UNSUPPORTED;
-  }
-  if (long_rows) {
+  if (hb->spot_nl == 16) return launch_spot_wide<MODE, 16>(hb, U, Y, out, s);
+  if (hb->spot_nl == 8) return launch_spot_wide<MODE, 8>(hb, U, Y, out, s);
+  if (hb->spot_maxl == 2 * kMAXLs) {
     switch (hb->spot_threads) {
       case 256: return launch_spot_t<MODE, 256, 4, 2 * kMAXLs>(hb, U, Y, out, s);
       case 512: return launch_spot_t<MODE, 512, 4, 2 * kMAXLs>(hb, U, Y, out, s);
@@ -1347,7 +1494,6 @@ int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *ou
     case 128: return launch_spot_t<MODE, 128, 4, kMAXLs>(hb, U, Y, out, s);
     case 256: return launch_spot_t<MODE, 256, 4, kMAXLs>(hb, U, Y, out, s);
     case 512: return launch_spot_t<MODE, 512, 4, kMAXLs>(hb, U, Y, out, s);
-    case 1024: return launch_spot_t<MODE, 1024, 4, kMAXLs>(hb, U, Y, out, s);
   }
   set_error("spot sweep: unsupported thread count");
   return FDADI_ERR_UNSUPPORTED;
@@ -1542,15 +1688,11 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return hb->h_opts[a].nS * hb->h_opts[a].nv > hb->h_opts[b].nS * hb->h_opts[b].nv;
   });
-  std::vector<Tile> st, vtl, ft;
+  std::vector<Tile> ft;
   for (int k : order) {
     const HOpt &o = hb->h_opts[k];
-    for (int64_t j0 = 0; j0 < o.nv; j0 += 4) st.push_back(Tile{k, (int32_t)j0});
-
     for (int64_t j0 = 0; j0 < o.nv; j0 += 64) ft.push_back(Tile{k, (int32_t)j0});
   }
-  hb->nstiles = (int)st.size();
-
   hb->nftiles = (int)ft.size();
   spot_shape(hb->nSmax, 0, 4, &hb->spot_threads, &hb->spot_maxl);
   hb->order = order;
@@ -1609,9 +1751,9 @@ int fdadi_hb_create(int32_t nopt, const fdadi_heston_option *opts, const double 
   up(hb->d_vars, vars, sizeof(double) * vo);
   up(hb->d_sp_row, sp_row.data(), sizeof(int64_t) * so);
   up(hb->d_vt_row, vt_row.data(), sizeof(int64_t) * vo);
-  up(hb->d_stiles, st.data(), sizeof(Tile) * st.size());
   up(hb->d_ftiles, ft.data(), sizeof(Tile) * ft.size());
   up(hb->d_base_dt, hb->base_dt.data(), sizeof(double) * nopt);
+  if (e == hipSuccess && build_spot_tiles(hb) != FDADI_OK) e = hipErrorUnknown;
   if (e == hipSuccess && build_var_tiles(hb) != FDADI_OK) e = hipErrorUnknown;
   if (e == hipSuccess && build_tables(hb, s) != FDADI_OK) e = hipErrorUnknown;
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1834,9 +1976,9 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
                 "fdadi_hb_set_shape: variance tiles follow from the lanes per line (pass 0)");
   if (kind == 0) {
     FDADI_REQUIRE(threads == 0 || threads == 64 || threads == 128 || threads == 256 ||
-                      threads == 512 || threads == 1024,
+                      threads == 512,
                   FDADI_ERR_ARG,
-                  "fdadi_hb_set_shape: threads must be a power of two in [64, 1024]");
+                  "fdadi_hb_set_shape: threads must be a power of two in [64, 512]");
     const int nl = lines == 0 ? 4 : lines;
     int t = 0, ml = 0;
     FDADI_RET(spot_shape(hb->nSmax, threads, nl, &t, &ml));

@@ -86,7 +86,8 @@ def test_graph_replay_equals_eager(fd):
 def test_launch_shapes_agree(fd):
     P, S, Vs, dts, O = draw(3, 3, nS_range=(40, 100), nv_range=(20, 60))
     outs = []
-    for ts, nl, tv in ((0, 0, 0), (256, 4, 16), (512, 4, 64), (256, 16, 8), (512, 16, 32)):
+    for ts, nl, tv in ((0, 0, 0), (256, 4, 16), (512, 4, 64), (256, 8, 16), (512, 8, 8),
+                       (256, 16, 8), (512, 16, 32)):
         hb = fd.HestonBatch(P, S, Vs, dts)
         if ts:
             hb.set_shape(0, ts, nl)
@@ -100,6 +101,44 @@ def test_launch_shapes_agree(fd):
     for x in outs[1:]:
         # chunking changes the summation order of the line solves only
         assert np.max(np.abs(x - outs[0])) <= 1e-12 * scale
+
+
+def test_every_column_residue_every_shape_matches_oracle(fd):
+    """Variance sizes 16..33 cover every residue modulo the tile widths (4, 8,
+    16): the last tile of an option then overlaps the one before it, and the
+    last column takes its j-2 neighbour from within its tile.  Craig-Sneyd,
+    10 steps, every spot launch shape against the oracle."""
+    rng = np.random.default_rng(11)
+    P, S, Vs, dts, O = [], [], [], [], []
+    for nv in range(16, 34):
+        nS = int(rng.integers(20, 44))
+        K, theta = 100 * rng.uniform(0.8, 1.2), rng.uniform(0.02, 0.1)
+        p = dict(r=0.03, kappa=2.0, theta=theta, sigma=0.4, rho=-0.6)
+        spots = sinh_space(K, 5 * K, 7.0, nS)
+        vars_ = exponential_space(0, theta, 5 * theta, 4.0, nv)
+        P.append(p)
+        S.append(spots)
+        Vs.append(vars_)
+        dts.append(1.0 / 10)
+        O.append(HestonOracle(HestonParams(spot=100, strike=K, r=p["r"], v0=theta, T=1.0,
+                                           kappa=p["kappa"], theta=theta, sigma=p["sigma"],
+                                           rho=p["rho"]), spots, vars_))
+    want = [o.solve_craigsneyd(10, dts[k], o.payoff(), theta=0.5) for k, o in enumerate(O)]
+    alts = [[a.solve_craigsneyd(10, dts[k], o.payoff(), theta=0.5) for a in alternates(o)]
+            for k, o in enumerate(O)]
+    for shape in ((0, 0), (64, 4), (128, 4), (256, 4), (256, 8), (512, 8), (256, 16),
+                  (512, 16)):
+        hb = fd.HestonBatch(P, S, Vs, dts)
+        if shape[0]:
+            hb.set_shape(0, *shape)
+        hb.set_state([o.payoff() for o in O])
+        hb.configure("cs", 0.5)
+        hb.step(10)
+        got = hb.get_state()
+        for k in range(len(O)):
+            tol, floor = grid_tolerance(want[k], alts[k])
+            err = float(np.max(np.abs(got[k] - want[k])))
+            assert err <= tol, (shape, k, got[k].shape, err, tol)
 
 
 def test_full_size_config1_steps(fd):
