@@ -428,21 +428,32 @@ struct RawRow<false> {
   static __device__ __forceinline__ Row4 zero() { return Row4{0.0, 0.0, 0.0, 0.0}; }
 };
 
+// (c A1 f)(i,j) with the scaled entries K of this column over row i of f (K.z
+// is zero except on the last column).
+__device__ __forceinline__ double a1s(const ColOp &K, const Row4 &w) {
+  return ((K.sup * w.r + K.mid * w.c) + K.sub * w.l) + K.z * w.m2;
+}
+
 // What the sliding window keeps of a row once it has entered: the value at
 // column j and the row's share of the cross product, wv . (x_{j-1}, x_j,
-// x_{j+1}), of U and of D = Y - U, and Y at column j.  The share is formed
-// once per row and used by the three windows that hold the row.
+// x_{j+1}), of U and of D = Y - U, Y at column j, and the A1 products the
+// stage takes at this row (dt A1 U, theta dt A1 U, dt A1 D, theta dt A1 Y).
+// Each is formed once per row, when the row enters.
 struct WinRow {
   double uc, urs, dc, drs, yc;
+  double f1u, t1u, f1d, t1y;
 };
 
 template <int MODE>
 __device__ __forceinline__ void enter_row(const VarCol &q, const Row4 &u, const Row4 &y,
-                                          WinRow &w, Row4 &d) {
+                                          WinRow &w) {
   constexpr bool HAS_Y = MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS;
   w.uc = u.c;
   w.urs = q.wv0 * u.l + q.wv1 * u.c + q.wv2 * u.r;
+  if constexpr (MODE != SM_IMPL) w.f1u = a1s(q.f1, u);
+  if constexpr (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) w.t1u = a1s(q.t1, u);
   if constexpr (HAS_Y) {
+    Row4 d;
     d.l = y.l - u.l;
     d.c = y.c - u.c;
     d.r = y.r - u.r;
@@ -450,13 +461,19 @@ __device__ __forceinline__ void enter_row(const VarCol &q, const Row4 &u, const 
     w.dc = d.c;
     w.drs = q.wv0 * d.l + q.wv1 * d.c + q.wv2 * d.r;
     w.yc = y.c;
+    if constexpr (MODE == SM_HV || MODE == SM_MCS) w.f1d = a1s(q.f1, d);
+    if constexpr (MODE == SM_HV) w.t1y = a1s(q.t1, y);
   }
 }
 
-// (c A1 f)(i,j) with the scaled entries K of this column over row i of f (K.z
-// is zero except on the last column).
-__device__ __forceinline__ double a1s(const ColOp &K, const Row4 &w) {
-  return ((K.sup * w.r + K.mid * w.c) + K.sub * w.l) + K.z * w.m2;
+// The S = 0 line (row 0 when the spot mesh starts at 0): A1 keeps only the
+// -r/2 of the split, c * (-r/2) * f(i, j).
+template <int MODE>
+__device__ __forceinline__ void zero_line(const VarCol &q, WinRow &w) {
+  if constexpr (MODE != SM_IMPL) w.f1u = q.f1.s0 * w.uc;
+  if constexpr (MODE == SM_FIRST || MODE == SM_CS || MODE == SM_MCS) w.t1u = q.t1.s0 * w.uc;
+  if constexpr (MODE == SM_HV || MODE == SM_MCS) w.f1d = q.f1.s0 * w.dc;
+  if constexpr (MODE == SM_HV) w.t1y = q.t1.s0 * w.yc;
 }
 
 // Work-group index -> tile index so that consecutive tiles (which share cache
@@ -553,7 +570,9 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   constexpr bool HAS_EX = MODE != SM_IMPL;
   constexpr bool TAB_LDS = NL != 4;
   constexpr int CHP = MAXL * kSpFields + 2;  // LDS table: doubles per chunk
-  constexpr int PF = 4;                      // state rows in flight
+  // state rows in flight (two for the second-stage modes on long chunks,
+  // whose window and registers are largest)
+  constexpr int PF = (MODE == SM_HV || MODE == SM_MCS) && MAXL == 16 ? 2 : 4;
   constexpr bool NB_DPP = true;              // neighbour columns by DPP
   double *cM = lds;
   double *cE = cM + NL * (TPL + 1);
@@ -630,18 +649,18 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   // Window: rows s-1 (a) and s (b) entered; rows s+1 .. s+PF in flight
   // (row s+1+m in ring slot m % PF).
   WinRow wa, wb, wc;
-  Row4 ub, db, yb, uc4, dc4, yc4;
   {
     const Row4 z{0.0, 0.0, 0.0, 0.0};
-    Row4 ua = load_row4(u, clamp_row(s - 1) * unv, lc), ya = z, da;
-    ub = load_row4(u, clamp_row(s) * unv, lc);
-    yb = z;
+    const Row4 ua = load_row4(u, clamp_row(s - 1) * unv, lc);
+    const Row4 ub = load_row4(u, clamp_row(s) * unv, lc);
+    Row4 ya = z, yb = z;
     if constexpr (HAS_Y) {
       ya = load_row4(y2, clamp_row(s - 1) * unv, lc);
       yb = load_row4(y2, clamp_row(s) * unv, lc);
     }
-    enter_row<MODE>(q, ua, ya, wa, da);
-    enter_row<MODE>(q, ub, yb, wb, db);
+    enter_row<MODE>(q, ua, ya, wa);
+    enter_row<MODE>(q, ub, yb, wb);
+    if (s0_zero) zero_line<MODE>(q, wb);
   }
   using RR = RawRow<NB_DPP>;
   typename RR::type ru[PF], ry[PF];
@@ -679,9 +698,7 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
       const Row4 un = expand_raw<NL>(ru[slot], jj, last);
       Row4 yn{0.0, 0.0, 0.0, 0.0};
       if constexpr (HAS_Y) yn = expand_raw<NL>(ry[slot], jj, last);
-      enter_row<MODE>(q, un, yn, wc, dc4);
-      uc4 = un;
-      yc4 = yn;
+      enter_row<MODE>(q, un, yn, wc);
       const unsigned rr = clamp_row(i + 1 + PF) * unv;
       ru[slot] = RR::load(u, rr, lc);
       if constexpr (HAS_Y) ry[slot] = RR::load(y2, rr, lc);
@@ -707,40 +724,34 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
       auto x01 = [&](double ra, double rb, double rc) {
         return q.rsv * (R.wsub * ra + R.wmid * rb + R.wsup * rc);
       };
-      // S = 0 line: A1 keeps only -r/2 (possible at row 0 only)
-      const bool zline = k == 0 && s0_zero;
-      auto a1 = [&](const ColOp &K, const Row4 &w) {
-        const double full = a1s(K, w);
-        return zline ? K.s0 * w.c : full;
-      };
       // Y0 = U + dt A0 U + dt R0 + dt A01 U + dt A1 U, in the reference's
       // order of its explicit operators (spot, cross, variance)
       const double y0 = ((wb.uc + (a0d(wa.uc, wb.uc, wc.uc) + dt * B.r0)) +
                          dt * x01(wa.urs, wb.urs, wc.urs)) +
-                        a1(q.f1, ub);
+                        wb.f1u;
       if constexpr (MODE == SM_FIRST) {
         rhs = y0 - a0t(wa.uc, wb.uc, wc.uc);
-        ex = a1(q.t1, ub);
+        ex = wb.t1u;
       } else if constexpr (MODE == SM_HV) {
         // the second-stage corrections act on the difference of the states
         // (FiniteDifferenceEngine.pyx:744-765, 818-835, 881-894)
         // V = Y0 - 1/2 F (U - Y2) over the explicit operators (no residual)
         const double v1 = ((y0 + 0.5 * a0d(wa.dc, wb.dc, wc.dc)) +
                            0.5 * (dt * x01(wa.drs, wb.drs, wc.drs))) +
-                          0.5 * a1(q.f1, db);
+                          0.5 * wb.f1d;
         rhs = v1 - a0t(wa.yc, wb.yc, wc.yc);
-        ex = a1(q.t1, yb);
+        ex = wb.t1y;
       } else if constexpr (MODE == SM_CS) {
         rhs = (y0 + (0.5 * dt) * x01(wa.drs, wb.drs, wc.drs)) - a0t(wa.uc, wb.uc, wc.uc);
-        ex = a1(q.t1, ub);
+        ex = wb.t1u;
       } else {  // SM_MCS: (1/2 - theta) F (Y2 - U), residual included
         const double h = 0.5 - o.th;
         const double xd = x01(wa.drs, wb.drs, wc.drs);
         const double v1 = (((y0 + tdt * xd) + h * (a0d(wa.dc, wb.dc, wc.dc) + dt * B.r0)) +
                            h * (dt * xd)) +
-                          h * a1(q.f1, db);
+                          h * wb.f1d;
         rhs = v1 - a0t(wa.uc, wb.uc, wc.uc);
-        ex = a1(q.t1, ub);
+        ex = wb.t1u;
       }
     }
     // the solve forces Dirichlet rows (0, and nS-1 under a barrier) to 0
@@ -758,11 +769,6 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     // slide the window
     wa = wb;
     wb = wc;
-    ub = uc4;
-    if constexpr (HAS_Y) {
-      db = dc4;
-      yb = yc4;
-    }
     if constexpr (!TAB_LDS) {
       ta0 = ta1; tb0 = tb1; ta1 = ta2; tb1 = tb2;
     }
