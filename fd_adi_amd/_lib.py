@@ -98,6 +98,7 @@ _SIGS = {
     "fdadi_hb_configure": (c_int, [c_vp, c_int, c_double, c_double, c_vp]),
     "fdadi_hb_configure_dev": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp]),
     "fdadi_hb_update": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "fdadi_hb_update_dev": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fdadi_hb_call_payoff": (c_int, [c_vp, c_vp, c_vp]),
     "fdadi_hb_read_bicubic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fdadi_hb_state": (c_int, [c_vp, _P(c_vp), _P(c_i64)]),
@@ -123,6 +124,18 @@ _SIGS = {
     "fdadi_mlp_predict_noalloc": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     "fdadi_mlp_gemm_nt": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
     "fdadi_heston_cf_call": (c_int, [c_vp, c_i64, c_vp, c_vp]),
+    "fdadi_forest_create": (c_int, [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                    c_double, _P(c_vp)]),
+    "fdadi_forest_destroy": (c_int, [c_vp]),
+    "fdadi_forest_predict": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "fdadi_serve_combine": (c_int, [c_vp, c_i64, c_vp, c_i32, c_vp, c_vp, c_vp, c_double, c_vp,
+                                    c_vp, c_vp, c_vp]),
+    "fdadi_serve_cf_rows": (c_int, [c_vp, c_i64, c_vp, c_vp]),
+    "fdadi_serve_gate": (c_int, [c_vp, c_vp, c_i64, c_double, c_vp, c_vp, c_vp, _P(c_i32),
+                                 c_vp]),
+    "fdadi_serve_label_inputs": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_vp,
+                                         c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fdadi_serve_scatter": (c_int, [c_vp, c_vp, c_i32, c_vp, c_vp]),
     "fdadi_gauss_legendre": (c_int, [c_i32, c_vp, c_vp]),
 }
 

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfdadi.so")
-SOURCES = ["array.hip", "tri.hip", "cross.hip", "heston.hip", "mlp.hip", "cf.hip"]
+SOURCES = ["array.hip", "tri.hip", "cross.hip", "heston.hip", "mlp.hip", "cf.hip", "serve.hip"]
 ARCH = os.environ.get("FDADI_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

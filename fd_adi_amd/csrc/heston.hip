@@ -1232,6 +1232,24 @@ __global__ void step_scalars_kernel(HOpt *opts, int nopt, const double *__restri
   }
 }
 
+// Parameters of every option from a device array of rows (r, kappa, theta,
+// sigma, rho, dt); the shapes and offsets stay.
+__global__ void set_params_kernel(HOpt *opts, int nopt, const double *__restrict__ rows,
+                                  double *base_dt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nopt) return;
+  const double *q = rows + 6 * (int64_t)k;
+  HOpt &o = opts[k];
+  o.r = q[0];
+  o.kappa = q[1];
+  o.theta = q[2];
+  o.sigma = q[3];
+  o.rho = q[4];
+  o.dt = q[5];
+  o.barrier = 0;
+  base_dt[k] = q[5];
+}
+
 // One work-group per spot row of every option: V(i, j) = max(S_i - K_k, 0).
 __global__ void call_payoff_kernel(const HOpt *opts, const int64_t *row_opt,
                                    const double *__restrict__ spots, int64_t sp_n,
@@ -1323,6 +1341,8 @@ struct fdadi_hb {
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
   bool dev_scalars = false;        // last configure took theta and dt from device memory
+  bool opts_on_device = false;     // parameters last set by fdadi_hb_update_dev (the host
+                                   // copy keeps only the shapes and offsets current)
   double *d_base_dt = nullptr;     // each option's step before the multiplier
   double *d_slots = nullptr;       // [theta, dt multiplier] of the configured scheme
   hipGraph_t graph = nullptr;
@@ -1605,8 +1625,10 @@ int build_tables(fdadi_hb *hb, hipStream_t s) {
 int refactor(fdadi_hb *hb, hipStream_t s, const double *d_theta = nullptr,
              const double *d_scale = nullptr) {
   const bool dev = d_theta || d_scale;
-  FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
-                               hipMemcpyHostToDevice, s));
+  // the device copy of the options is current after fdadi_hb_update_dev
+  if (!hb->opts_on_device)
+    FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
+                                 hipMemcpyHostToDevice, s));
   const bool reuse = d_theta == hb->d_slots;  // refactorisation with the kept pair
   hipLaunchKernelGGL(step_scalars_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0, s,
                      hb->d_opts, hb->nopt, hb->d_base_dt, d_theta, d_scale, hb->theta,
@@ -1852,6 +1874,7 @@ int fdadi_hb_update(fdadi_hb *hb, int32_t nopt, const fdadi_heston_option *opts,
   // The host arrays may be pageable: synchronous copies, after the stream's
   // earlier work (which may still read the tables) has finished.
   FDADI_TRY_HIP(hipStreamSynchronize(s));
+  hb->opts_on_device = false;
   FDADI_TRY_HIP(hipMemcpy(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * nopt,
                           hipMemcpyHostToDevice));
   FDADI_TRY_HIP(hipMemcpy(hb->d_spots, spots, sizeof(double) * hb->sp_n, hipMemcpyHostToDevice));
@@ -1866,6 +1889,26 @@ int fdadi_hb_update(fdadi_hb *hb, int32_t nopt, const fdadi_heston_option *opts,
       FDADI_RET(refactor(hb, s));
   }
   FDADI_TRY_HIP(hipStreamSynchronize(s));
+  return FDADI_OK;
+}
+
+int fdadi_hb_update_dev(fdadi_hb *hb, const double *d_params, const double *d_spots,
+                        const double *d_vars, void *stream) {
+  FDADI_REQUIRE(hb && d_params, FDADI_ERR_ARG, "fdadi_hb_update_dev: bad arguments");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(set_params_kernel, dim3(grid_for(hb->nopt, 64)), dim3(64), 0, s,
+                     hb->d_opts, hb->nopt, d_params, hb->d_base_dt);
+  FDADI_CHECK_LAUNCH();
+  if (d_spots)
+    FDADI_TRY_HIP(hipMemcpyAsync(hb->d_spots, d_spots, sizeof(double) * hb->sp_n,
+                                 hipMemcpyDeviceToDevice, s));
+  if (d_vars)
+    FDADI_TRY_HIP(hipMemcpyAsync(hb->d_vars, d_vars, sizeof(double) * hb->vt_n,
+                                 hipMemcpyDeviceToDevice, s));
+  hb->opts_on_device = true;
+  FDADI_RET(build_tables(hb, s));
+  // theta and the dt multiplier of the configured scheme stay in d_slots
+  if (hb->scheme >= 0) FDADI_RET(refactor(hb, s, hb->d_slots, hb->d_slots + 1));
   return FDADI_OK;
 }
 

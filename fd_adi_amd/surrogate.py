@@ -67,6 +67,99 @@ class DeviceInts(object):
         base = self.buf.p.value if hasattr(self.buf.p, "value") else int(self.buf.p)
         return ctypes.c_void_p(base + 4 * int(k))
 
+    def to_numpy(self):
+        return self.buf.to_numpy().view(np.int32)[:self.n].copy()
+
+
+class _At(object):
+    """A device address with the ``p`` of SizedArrayPtr (a view into a buffer)."""
+
+    __slots__ = ("p",)
+
+    def __init__(self, addr):
+        self.p = ctypes.c_void_p(int(addr))
+
+
+class _IntView(object):
+    """int32 view of a device buffer with the ``at`` of DeviceInts."""
+
+    def __init__(self, buf):
+        self.buf = buf
+        self.p = buf.p
+
+    def at(self, k):
+        return ctypes.c_void_p(_addr(self.buf) + 4 * int(k))
+
+
+def _addr(buf):
+    p = buf.p
+    return p.value if hasattr(p, "value") else int(p or 0)
+
+
+def _offset(buf, k, itemsize=8):
+    """View of ``buf`` starting at element k."""
+    return _At(_addr(buf) + itemsize * int(k))
+
+
+class Workspace(object):
+    """Device scratch buffers kept across calls, grown on demand (sizes in
+    doubles)."""
+
+    def __init__(self):
+        self._bufs = {}
+
+    def get(self, name, n):
+        b = self._bufs.get(name)
+        if b is None or b.size < n:
+            b = SizedArrayPtr().alloc(max(int(n), 1))
+            self._bufs[name] = b
+        return b
+
+
+class Forest(object):
+    """The regression trees of a fitted scikit-learn
+    ``HistGradientBoostingRegressor`` on the device (csrc/serve.hip): the
+    gate's error model evaluated where the queries are.  Prediction is the
+    model's baseline plus the reached leaves, tree by tree (scikit-learn's
+    order)."""
+
+    def __init__(self, model, nfeat):
+        trees = [it[0].nodes for it in model._predictors]
+        nodes = np.concatenate(trees)
+        if nodes["is_categorical"].any():
+            raise NotImplementedError("Forest: categorical splits are not supported")
+        start = np.zeros(len(trees) + 1, dtype=np.int64)
+        start[1:] = np.cumsum([t.shape[0] for t in trees])
+        self._keep = [np.ascontiguousarray(a) for a in (
+            start, nodes["feature_idx"].astype(np.int32), nodes["num_threshold"].astype(float),
+            nodes["left"].astype(np.int32), nodes["right"].astype(np.int32),
+            nodes["is_leaf"].astype(np.uint8), nodes["value"].astype(float))]
+        self.base = float(np.ravel(model._baseline_prediction)[0])
+        self.nfeat = int(nfeat)
+        h = ctypes.c_void_p()
+        _lib.call("fdadi_forest_create", len(trees), *[_lib.ptr(a) for a in self._keep[:1]],
+                  self.nfeat, *[_lib.ptr(a) for a in self._keep[1:]], self.base,
+                  ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib.fdadi_forest_destroy(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def predict_device(self, X, n, out):
+        _lib.call("fdadi_forest_predict", self._h, X.p, int(n), out.p, _lib.default_stream())
+
+    def predict(self, X_host):
+        X_host = np.ascontiguousarray(X_host, dtype=float)
+        n = X_host.shape[0]
+        X, out = SizedArrayPtr(X_host), SizedArrayPtr().alloc(max(n, 1))
+        self.predict_device(X, n, out)
+        return out.to_numpy()[:n]
+
 
 def log_features(P):
     """Monotone per-feature map: the logarithm of the positive scale
@@ -355,6 +448,18 @@ class AdiLabeler(object):
         instead of the graph (the same kernels): ``last_stage_ms`` then holds
         the device milliseconds per stage slot summed over the nt steps,
         ``last_batch`` that batch's option count."""
+        P = np.ascontiguousarray(P, dtype=float)
+        n = P.shape[0]
+        if n == 0:
+            return np.zeros(0)
+        Pd, out = SizedArrayPtr(P), SizedArrayPtr().alloc(n)
+        self.price_rows_device(Pd, None, n, out, timed=timed)
+        return out.to_numpy()[:n]
+
+    def price_host_batches(self, P, timed=False):
+        """``price`` with each batch's parameters and meshes built on the host
+        and loaded by ``HestonBatch.update`` (the engine's host path; kept for
+        tests of that path)."""
         P = np.asarray(P, dtype=float)
         out = np.empty(P.shape[0])
         c0 = 0
@@ -381,6 +486,49 @@ class AdiLabeler(object):
             out[c0:c0 + m] = res.to_numpy()[:m]
             c0 += m
         return out
+
+    def price_rows_device(self, P, idx, m, out, timed=False):
+        """FP64 ADI prices of rows idx[0..m) (a ``DeviceInts``; None: rows
+        0..m) of the device feature rows P, scattered into out[idx[i]] (out[i]
+        without idx); nothing but the launches leaves the host.  ``timed`` as
+        in ``price`` (the first batch of the largest tier, else the last)."""
+        if m <= 0:
+            return
+        if not hasattr(self, "_unit"):
+            self._unit = SizedArrayPtr(np.ascontiguousarray(_unit_spot_mesh(self.nS)))
+        s = _lib.default_stream()
+        c0, pending = 0, bool(timed)
+        while c0 < m:
+            cap = self._tier(m - c0)
+            mm = min(cap, m - c0)
+            hb, (K, S0, v0, res) = self._engine(cap)
+            ws = self._dev_ws(cap)
+            ip = None if idx is None else idx.at(c0)
+            Pc = P if idx is not None else _offset(P, 8 * c0)
+            _lib.call("fdadi_serve_label_inputs", Pc.p, ip, int(mm), int(cap), self.nt,
+                      self._unit.p, self.nS, ws["params"].p, ws["spots"].p, K.p, S0.p, v0.p, s)
+            _lib.call("fdadi_hb_update_dev", hb._h, ws["params"].p, ws["spots"].p, None, s)
+            hb.call_payoff(K)
+            if pending and (cap == self.tiers[-1] or c0 + mm >= m):
+                self.last_stage_ms = hb.step_timed(self.nt)
+                self.last_batch = cap
+                pending = False
+            else:
+                hb.replay()
+            hb.read_bicubic(S0, v0, res)
+            if idx is None:
+                _lib.call("fdadi_array_copy", _offset(out, c0).p, res.p, int(mm), s)
+            else:
+                _lib.call("fdadi_serve_scatter", res.p, ip, int(mm), out.p, s)
+            c0 += mm
+
+    def _dev_ws(self, cap):
+        if not hasattr(self, "_ws"):
+            self._ws = {}
+        if cap not in self._ws:
+            self._ws[cap] = {"params": SizedArrayPtr().alloc(6 * cap),
+                             "spots": SizedArrayPtr().alloc(self.nS * cap)}
+        return self._ws[cap]
 
     def grids(self, P):
         """Value grids [n, nS, nv] of feature rows P (one tier-sized batch at
@@ -605,6 +753,12 @@ class ErrorModel(object):
     def __call__(self, P, spread):
         return self.model.predict(self.features(P, spread))
 
+    def forest(self):
+        """The fitted trees on the device (``Forest``; kind "trees" only)."""
+        if self.kind != "trees":
+            raise NotImplementedError("ErrorModel.forest: the trees model only")
+        return Forest(self.model, 10)
+
 
 def fit_surrogate(P, y, members=4, steps=16000, batch=8192, fmap="identity", atol=1e-4,
                   max_fail=0.0, holdout=0.25, calib=1.0 / 3.0, error_steps=4000, seed=0,
@@ -681,7 +835,18 @@ class SurrogatePricer(object):
         return inside & (feller_ratio(P) >= self.feller_min) & (score <= self.tau)
 
     def price(self, P, **adi_kw):
+        """Prices and the admitted flags of host feature rows P.  With the
+        trees error model and the default fallback the batch goes through
+        ``price_device``; otherwise the gate runs on the host."""
         P = np.asarray(P, dtype=float)
+        if self.device_ready():
+            n = P.shape[0]
+            if n == 0:
+                return np.zeros(0), np.zeros(0, dtype=bool)
+            Pd = SizedArrayPtr(np.ascontiguousarray(P))
+            out, ok = SizedArrayPtr().alloc(n), DeviceInts(np.zeros(n, dtype=np.int32))
+            self.price_device(Pd, n, out, ok, **adi_kw)
+            return out.to_numpy()[:n], ok.to_numpy()[:n].astype(bool)
         mean, spread = self.ensemble.predict(P)
         ok = self.gate(P, self.score(P, spread))
         out = mean.copy()
@@ -689,3 +854,47 @@ class SurrogatePricer(object):
         if (~ok).any():
             out[~ok] = self.adi(P[~ok], **adi_kw)
         return out, ok
+
+    def device_ready(self):
+        """Whether the whole batch can be served on the device: the trees
+        error model, identity-mapped members on the semi-analytic baseline,
+        and the default ADI fallback."""
+        em = self.error_model
+        return (isinstance(em, ErrorModel) and em.kind == "trees" and self.adi is adi_prices and
+                self.ensemble.base == "cf" and
+                all(m.fmap_name == "identity" for m in self.ensemble.members))
+
+    def price_device(self, P, n, out, ok, timed=False, labeler_kw=None):
+        """Answer n feature rows P (device, 8 doubles each) into out (device,
+        n prices) and ok (a ``DeviceInts`` of n admitted flags): the members'
+        corrections, the semi-analytic baseline, the gate's features and
+        trees, the gate and the FP64 ADI fallback of the rejected rows all run
+        on the device; only the count of rejected rows comes back.
+        ``timed``: the fallback's first batch is stepped stage by stage with
+        HIP events (``AdiLabeler.price``)."""
+        if not self.device_ready():
+            raise NotImplementedError("price_device: needs the trees error model, "
+                                      "identity-mapped members on the cf baseline and the "
+                                      "default fallback")
+        if not hasattr(self, "_ws"):
+            self._ws = Workspace()
+            self._forest = self.error_model.forest()
+        ws, M, s = self._ws, len(self.ensemble.members), _lib.default_stream()
+        Y, base = ws.get("Y", M * n), ws.get("base", n)
+        rows, X, score = ws.get("rows", 9 * n), ws.get("X", 10 * n), ws.get("score", n)
+        inside, idx = ws.get("inside", (n + 1) // 2), ws.get("idx", (n + 1) // 2)
+        work = ws.get("work", (n + 255) // 256 + 2)
+        for k, m in enumerate(self.ensemble.members):
+            m.predict_device(P, n, _offset(Y, k * n))
+        _lib.call("fdadi_serve_cf_rows", P.p, int(n), rows.p, s)
+        _lib.call("fdadi_heston_cf_call", rows.p, int(n), base.p, s)
+        lo, hi = np.ascontiguousarray(self.lo), np.ascontiguousarray(self.hi)
+        _lib.call("fdadi_serve_combine", P.p, int(n), Y.p, M, base.p, _lib.ptr(lo), _lib.ptr(hi),
+                  self.feller_min, out.p, X.p, inside.p, s)
+        self._forest.predict_device(X, n, score)
+        nrej = _lib.c_i32()
+        _lib.call("fdadi_serve_gate", score.p, inside.p, int(n), self.tau, ok.p, idx.p, work.p,
+                  ctypes.byref(nrej), s)
+        self.last_fallback = int(nrej.value)
+        lab = labeler(**(labeler_kw or {}))
+        lab.price_rows_device(P, _IntView(idx), self.last_fallback, out, timed=timed)

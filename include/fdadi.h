@@ -207,6 +207,15 @@ int fdadi_hb_configure_dev(fdadi_hb *hb, int scheme, const double *d_theta,
  * option by run_parameters.py:16-36).  Synchronises. */
 int fdadi_hb_update(fdadi_hb *hb, int32_t nopt, const fdadi_heston_option *opts,
                     const double *spots, const double *vars, void *stream);
+/* fdadi_hb_update from device memory, without synchronising: d_params holds
+ * nopt rows (r, kappa, theta, sigma, rho, dt); d_spots / d_vars the packed
+ * meshes (sp_n / vt_n doubles, same shapes), or NULL to keep the current
+ * ones.  Barriers are cleared.  The labeller and the surrogate's fallback use
+ * it to price a device-resident batch (no counterpart in the reference, which
+ * builds a new engine per option from host values,
+ * FiniteDifferenceEngineGPU.pyx:603-694). */
+int fdadi_hb_update_dev(fdadi_hb *hb, const double *d_params, const double *d_spots,
+                        const double *d_vars, void *stream);
 /* European call payoff max(S - K_k, 0) on every option's grid (d_strikes:
  * device array of nopt strikes); the initial condition of
  * FiniteDifferenceEngineGPU.pyx:682. */
@@ -308,6 +317,58 @@ int fdadi_heston_cf_call(const double *d_rows, int64_t n, double *d_out, void *s
 /* Gauss-Legendre nodes x[n] (ascending) and weights w[n] on (-1, 1), host only
  * (the quadrature rule of fdadi_heston_cf_call). */
 int fdadi_gauss_legendre(int32_t n, double *x, double *w);
+
+/* ---- device-resident serving of the surrogate ----------------------------------
+ * No counterpart in the reference (its pricers are the ADI engines and the COS
+ * method, heston.py:490-693).  A batch of feature rows in device memory (8
+ * doubles each: kappa, theta, sigma, rho, v0, r, T, K/S0; S0 = 100) is
+ * answered without a host round trip; only the number of fallback rows comes
+ * back.
+ *
+ * Regression trees (the gate's error model): ntrees trees, tree t holding
+ * nodes [tree_start[t], tree_start[t+1]); per node the feature, threshold,
+ * leaf flag, leaf value and the children as indices local to the tree (x goes
+ * left when x[feature] <= threshold).  Children must come after their parent.
+ * Prediction = base + the sum of the reached leaves, tree by tree in order
+ * (the summation order of scikit-learn's gradient boosting). */
+typedef struct fdadi_forest fdadi_forest;
+int fdadi_forest_create(int32_t ntrees, const int64_t *tree_start, int32_t nfeat,
+                        const int32_t *feature, const double *threshold, const int32_t *left,
+                        const int32_t *right, const uint8_t *is_leaf, const double *value,
+                        double base, fdadi_forest **out);
+int fdadi_forest_destroy(fdadi_forest *f);
+/* d_X: n rows of nfeat doubles; d_out[n]. */
+int fdadi_forest_predict(const fdadi_forest *f, const double *d_X, int64_t n, double *d_out,
+                         void *stream);
+/* price = d_base + mean of the M member corrections d_Y (M x n, member-major);
+ * d_X (n x 10) = the error model's features (log kappa, log theta, log sigma,
+ * rho, log v0, r, log T, log K/S0, log Feller ratio 2 kappa theta / sigma^2,
+ * log10(spread + 1e-12), spread the members' standard deviation with M - 1);
+ * d_inside = row within the box [lo, hi] (host arrays of 8) with Feller ratio
+ * >= feller_min. */
+int fdadi_serve_combine(const double *d_P, int64_t n, const double *d_Y, int32_t M,
+                        const double *d_base, const double *lo, const double *hi,
+                        double feller_min, double *d_price, double *d_X, int32_t *d_inside,
+                        void *stream);
+/* Rows of fdadi_heston_cf_call (9 doubles each) from feature rows. */
+int fdadi_serve_cf_rows(const double *d_P, int64_t n, double *d_rows, void *stream);
+/* d_ok = inside && score <= tau; the rejected rows' indices in ascending order
+ * into d_idx (room for n), their count into *n_rejected (synchronises).
+ * d_work: ceil(n / 256) + 1 ints of scratch. */
+int fdadi_serve_gate(const double *d_score, const int32_t *d_inside, int64_t n, double tau,
+                     int32_t *d_ok, int32_t *d_idx, int32_t *d_work, int32_t *n_rejected,
+                     void *stream);
+/* Inputs of a fallback batch of cap options for rows d_idx[0..m) of d_P (d_idx
+ * NULL: rows 0..m); rows m..cap-1 repeat the last one: d_params (cap x 6:
+ * r, kappa, theta, sigma, rho, T / nt) for fdadi_hb_update_dev, d_spots (cap x
+ * nS) = K * d_unit, the strikes, S0 and v0 of the payoff and the read-out. */
+int fdadi_serve_label_inputs(const double *d_P, const int32_t *d_idx, int32_t m, int32_t cap,
+                             int32_t nt, const double *d_unit, int32_t nS, double *d_params,
+                             double *d_spots, double *d_K, double *d_S0, double *d_v0,
+                             void *stream);
+/* d_dst[d_idx[i]] = d_src[i], i < m. */
+int fdadi_serve_scatter(const double *d_src, const int32_t *d_idx, int32_t m, double *d_dst,
+                        void *stream);
 
 #ifdef __cplusplus
 }
