@@ -386,8 +386,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--config1-steps", type=int, default=5, help="timed 2048x1024 solves")
     ap.add_argument("--surrogate-labels", type=int, default=1 << 20)
-    ap.add_argument("--surrogate-steps", type=int, default=16000)
-    ap.add_argument("--surrogate-members", type=int, default=4)
+    ap.add_argument("--surrogate-steps", type=int, default=48000)
+    ap.add_argument("--surrogate-members", type=int, default=8)
     ap.add_argument("--gate-max-fail", type=float, default=0.0,
                     help="failure budget of the gate's calibration (0: no calibration query "
                          "beyond 1e-4 is admitted)")
@@ -520,7 +520,8 @@ def main():
         cfg = {"workload": workload, "queries": nq, "queries_per_step": args.queries_per_step,
                "outside_box_frac": 0.05,
                "labels": sur["labels"], "members": sur["members"],
-               "gate": "error model, calibrated at %g failures above 1e-4" % sur["gate_max_fail"],
+               "gate": "error model (regression trees on the device), calibrated at %g failures "
+                       "above 0.9e-4" % sur["gate_max_fail"],
                "fallback": "batched FP64 ADI, 256x128, nt=200, Craig-Sneyd"}
         steps, warmup = args.steps, args.warmup
     else:
