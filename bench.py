@@ -259,6 +259,9 @@ def surrogate_run(n_labels, q_per_step, K, W, steps, members, max_fail, check, s
         out["admitted_max_abs_err"] = float(err.max())
         out["admitted_mean_abs_err"] = float(err.mean())
         out["admitted_beyond_atol"] = int((err > 1e-4).sum())
+        out["admitted_frac_beyond_atol"] = float(np.mean(err > 1e-4))
+        out["admitted_err_quantiles"] = {"%g" % q: float(np.quantile(err, q))
+                                         for q in (0.5, 0.99, 0.9999)}
     return out
 
 
@@ -391,7 +394,7 @@ def main():
     ap.add_argument("--gate-max-fail", type=float, default=0.0,
                     help="failure budget of the gate's calibration (0: no calibration query "
                          "beyond 1e-4 is admitted)")
-    ap.add_argument("--check-admitted", type=int, default=32768,
+    ap.add_argument("--check-admitted", type=int, default=131072,
                     help="admitted answers checked against FP64 ADI (-1: all of them)")
     ap.add_argument("--probe", action="store_true",
                     help="profiling probe: 20 HV steps of the 2048x1024 option, then 5 "
