@@ -43,19 +43,35 @@ constexpr int kLdsPitch = kKStep + 8;  // bf16 elements per LDS row (16 B pad)
 
 enum Epi { EPI_FWD = 0, EPI_INF = 1, EPI_BWD = 2, EPI_F32 = 3 };
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
-__device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
-  return s * (1.0f + z * (1.0f - s));
+// Logistic sigmoid with the hardware reciprocal (about 1 ulp; every result
+// that uses it is rounded to bf16).  silu(z) = z s(z), silu'(z) = s (1 + z (1 - s)).
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
 }
 
 struct EpiArgs {
   const float *bias;  // [N] (EPI_FWD/EPI_INF)
-  float *Z;           // [M x N] pre-activations (EPI_FWD writes, EPI_BWD reads)
+  bf16 *dT;           // [N x M] SiLU derivative at the pre-activations, transposed
+                      // (EPI_FWD writes, EPI_BWD reads)
   bf16 *out;          // [M x N] bf16 result
-  bf16 *outT;         // [N x M] bf16 transposed result (may be null)
-  float *C;           // [M x N] fp32 result (EPI_F32)
+  bf16 *outT;         // [N x M] bf16 transposed result (EPI_FWD, EPI_BWD)
+  float *C;           // [M x N] fp32 result (EPI_F32); with split-K the
+                      // [splits x M x N] partial products
+  const bf16 *w4;     // EPI_FWD, last hidden layer: the head's weights [N] ...
+  float *ypart;       // ... and its partial outputs [N/128 x M], one per column tile
 };
+
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+constexpr int kEpPitch = kTile + 8;  // bf16 elements per row of an epilogue tile
+// LDS of the GEMM: the K loop's A and B buffers; the epilogue tiles reuse it
+// (one 128 x 128 tile for EPI_INF, two for EPI_FWD and EPI_BWD).
+constexpr int kLoopLds = 4 * kTile * kLdsPitch;
+constexpr int kTileLds = kTile * kEpPitch;
+constexpr int gemm_lds(int epi) {
+  return epi == EPI_F32 ? kLoopLds
+         : epi == EPI_INF ? (kLoopLds > kTileLds ? kLoopLds : kTileLds)
+                          : (kLoopLds > 2 * kTileLds ? kLoopLds : 2 * kTileLds);
+}
 
 // C = A B^T on MFMA 16x16x32 bf16.  256 threads = 4 waves in a 2 x 2 grid,
 // each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA blocks.  The K loop stages
@@ -63,51 +79,63 @@ struct EpiArgs {
 // MFMA operand layout (v_mfma_f32_16x16x32_bf16): lane l holds
 //   A[i = l%16][k = 8*(l/16) .. +7],  B[k = 8*(l/16) .. +7][j = l%16],
 //   D[i = 4*(l/16) + t][j = l%16] for t = 0..3.
+// The grids here are about one work-group per CU, one wave per SIMD, so the
+// loads of a K step must be in flight long before it: a ring of kDepth
+// register stages keeps the next kDepth - 1 steps' global loads outstanding
+// while the MFMAs of the current step run (16 MFMAs of a wave take ~256
+// cycles, an L2 load several hundred).  The K extent of a work-group is a
+// multiple of kDepth steps, so the ring needs no remainder code.
+constexpr int kDepth = 4;
+constexpr int kKBlock = kKStep * kDepth;  // K granularity of the GEMMs
+
 template <int EPI>
 __global__ void __launch_bounds__(256)
 gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, int N, int K,
                EpiArgs ep) {
-  // Two LDS buffers: the MFMAs of K step k read buffer k % 2 while the global
-  // loads of step k + 1 are in flight; those then go to the other buffer, so
-  // one barrier per step suffices.
-  __shared__ __attribute__((aligned(16))) bf16 As[2][kTile * kLdsPitch];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[2][kTile * kLdsPitch];
+  // Two LDS buffers: the MFMAs of K step k read buffer k % 2 while step
+  // k + 1 is written to the other one, so one barrier per step suffices.
+  __shared__ __attribute__((aligned(16))) bf16 smem[gemm_lds(EPI)];
+  bf16 *const As[2] = {smem, smem + kTile * kLdsPitch};
+  bf16 *const Bs[2] = {smem + 2 * kTile * kLdsPitch, smem + 3 * kTile * kLdsPitch};
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int wm = wave / 2, wn = wave % 2;
-  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  // Work-groups are dealt to the 8 XCDs round-robin by linear id; renumber
+  // them so that each XCD gets a contiguous run of tiles (x fastest): the
+  // N tiles of one row block of A, and of one split, then share that XCD's L2.
+  const int gx = gridDim.x, gy = gridDim.y, total = gx * gy * (int)gridDim.z;
+  int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  if (total % 8 == 0) L = (L % 8) * (total / 8) + L / 8;
+  const int bx = L % gx, by = (L / gx) % gy, bz = L / (gx * gy);
+  const int m0 = by * kTile, n0 = bx * kTile;
   // split-K: work-group z covers K columns [z*Ks, (z+1)*Ks) (gridDim.z = 1: all)
-  const int Ks = K / (int)gridDim.z, kbeg = (int)blockIdx.z * Ks, kend = kbeg + Ks;
+  const int Ks = K / (int)gridDim.z, kbeg = bz * Ks, nsteps = Ks / kKStep;
   f32x4 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // Global -> LDS: each thread moves 2 x 16 B of A and of B per K step.
+  // Global -> registers -> LDS: each thread moves 2 x 16 B of A and of B per K step.
   const int lrow = tid / 4, lk = (tid % 4) * 8;
-  bf16x8 ra[2], rb[2];
-  auto fetch = [&](int k0) {
+  const bf16 *Ag = A + (size_t)(m0 + lrow) * K + kbeg + lk;
+  const bf16 *Bg = B + (size_t)(n0 + lrow) * K + kbeg + lk;
+  bf16x8 ra[kDepth][2], rb[kDepth][2];
+  auto fetch = [&](bf16x8 *xa, bf16x8 *xb, int step) {
+    const int k = step * kKStep;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      xa[p] = *reinterpret_cast<const bf16x8 *>(Ag + (size_t)64 * p * K + k);
+      xb[p] = *reinterpret_cast<const bf16x8 *>(Bg + (size_t)64 * p * K + k);
+    }
+  };
+  auto stage = [&](const bf16x8 *xa, const bf16x8 *xb, int buf) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       const int r = lrow + 64 * p;
-      ra[p] = *reinterpret_cast<const bf16x8 *>(A + (size_t)(m0 + r) * K + k0 + lk);
-      rb[p] = *reinterpret_cast<const bf16x8 *>(B + (size_t)(n0 + r) * K + k0 + lk);
+      *reinterpret_cast<bf16x8 *>(As[buf] + r * kLdsPitch + lk) = xa[p];
+      *reinterpret_cast<bf16x8 *>(Bs[buf] + r * kLdsPitch + lk) = xb[p];
     }
   };
-  auto stage = [&](int buf) {
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int r = lrow + 64 * p;
-      *reinterpret_cast<bf16x8 *>(As[buf] + r * kLdsPitch + lk) = ra[p];
-      *reinterpret_cast<bf16x8 *>(Bs[buf] + r * kLdsPitch + lk) = rb[p];
-    }
-  };
-  fetch(kbeg);
-  stage(0);
-  __syncthreads();
-  int buf = 0;
-  for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
-    const bool more = k0 + kKStep < kend;
-    if (more) fetch(k0 + kKStep);
+  auto compute = [&](int buf) {
     bf16x8 af[4], bfr[4];
     const int kseg = (lane / 16) * 8;
 #pragma unroll
@@ -125,65 +153,177 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
 #pragma unroll
       for (int b = 0; b < 4; ++b)
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
-    if (more) stage(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
+  };
+#pragma unroll
+  for (int j = 0; j < kDepth; ++j) fetch(ra[j], rb[j], j);
+  stage(ra[0], rb[0], 0);
+  __syncthreads();
+  // Step s computes from LDS buffer s % 2; its registers (ring slot s %
+  // kDepth) were staged one step earlier and now take step s + kDepth.  The
+  // loop body is free of branches (past the end the loads repeat the last
+  // step and the stage is never read), so the compiler's wait counts keep
+  // kDepth - 1 steps of loads in flight.
+  for (int s0 = 0; s0 < nsteps; s0 += kDepth) {
+#pragma unroll
+    for (int j = 0; j < kDepth; ++j) {
+      const int st = s0 + j;
+      fetch(ra[j], rb[j], min(st + kDepth, nsteps - 1));
+      compute(j & 1);
+      stage(ra[(j + 1) % kDepth], rb[(j + 1) % kDepth], (j + 1) & 1);
+      __syncthreads();
+    }
   }
-  // Epilogue.
+  // Epilogue.  Lane l of wave (wm, wn) holds, for MFMA block (a, b), rows
+  // ml = wm*64 + a*16 + 4*(l/16) + t (t = 0..3) of column nl = wn*64 + b*16 + l%16.
+  if (EPI == EPI_F32) {
+    // split z writes its partial product to slice z of C
+    float *Cz = ep.C + (size_t)bz * M * N;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int n = n0 + wn * 64 + b * 16 + lane % 16;
+        const int mb = m0 + wm * 64 + a * 16 + 4 * (lane / 16);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) Cz[(size_t)(mb + t) * N + n] = acc[a][b][t];
+      }
+    return;
+  }
+  // The bf16 results go through LDS tiles so that every global access is a
+  // coalesced 16-byte one: T [ml][nl] (row-major) for out, TT [nl][ml] for
+  // the transposed outT and dT.  The K loop ended with a barrier, so its
+  // buffers are free.
+  bf16 *const T = smem, *const TT = smem + kTileLds;
+  // 2048 chunks of 8 elements per tile, 8 per thread: chunk q is row q / 16,
+  // elements 8 (q % 16) .. +7 of that row (16 lanes cover one 256-byte row).
+  auto tile_to_global = [&](const bf16 *tile, bf16 *g, int ld, int r0, int c0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int q = tid + 256 * i, r = q / 16, c = (q % 16) * 8;
+      *reinterpret_cast<bf16x8 *>(g + (size_t)(r0 + r) * ld + c0 + c) =
+          *reinterpret_cast<const bf16x8 *>(tile + r * kEpPitch + c);
+    }
+  };
+  float r[4][4][4];
+  if (EPI == EPI_BWD) {
+    // r = (dZ_next W) * silu'(z): the derivative tile dT [n0.., m0..] into TT
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int q = tid + 256 * i, nl = q / 16, c = (q % 16) * 8;
+      *reinterpret_cast<bf16x8 *>(TT + nl * kEpPitch + c) =
+          *reinterpret_cast<const bf16x8 *>(ep.dT + (size_t)(n0 + nl) * M + m0 + c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int nl = wn * 64 + b * 16 + lane % 16, ml = wm * 64 + a * 16 + 4 * (lane / 16);
+        const bf16x4 d = *reinterpret_cast<const bf16x4 *>(TT + nl * kEpPitch + ml);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) r[a][b][t] = acc[a][b][t] * (float)d[t];
+      }
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const float bn = ep.bias[n0 + wn * 64 + b * 16 + lane % 16];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) r[a][b][t] = acc[a][b][t] + bn;  // pre-activation z
+    }
+  }
+  // activations (or back-propagated gradients) into T, and into TT
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int n = n0 + wn * 64 + b * 16 + lane % 16;
-      const int mb = m0 + wm * 64 + a * 16 + 4 * (lane / 16);
-      float v[4] = {acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
-      if (EPI == EPI_F32) {
-        // split z writes its partial product to slice z of C
-        float *Cz = ep.C + (size_t)blockIdx.z * M * N;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) Cz[(size_t)(mb + t) * N + n] = v[t];
-        continue;
-      }
-      bf16 o[4];
+      const int nl = wn * 64 + b * 16 + lane % 16, ml = wm * 64 + a * 16 + 4 * (lane / 16);
+      bf16x4 o;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const size_t idx = (size_t)(mb + t) * N + n;
-        float r;
-        if (EPI == EPI_BWD) {
-          r = v[t] * dsilu(ep.Z[idx]);
-        } else {
-          const float z = v[t] + ep.bias[n];
-          if (EPI == EPI_FWD) ep.Z[idx] = z;
-          r = silu(z);
-        }
-        o[t] = static_cast<bf16>(r);
-        ep.out[idx] = o[t];
+        o[t] = static_cast<bf16>(EPI == EPI_BWD ? r[a][b][t] : r[a][b][t] * sigmoid(r[a][b][t]));
+        T[(ml + t) * kEpPitch + nl] = o[t];
       }
-      if (ep.outT) {
-        // four consecutive rows of one column -> 8 contiguous bytes of the transpose
-        typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
-        bf16x4 w = {o[0], o[1], o[2], o[3]};
-        *reinterpret_cast<bf16x4 *>(ep.outT + (size_t)n * M + mb) = w;
-      }
+      if (EPI != EPI_INF) *reinterpret_cast<bf16x4 *>(TT + nl * kEpPitch + ml) = o;
     }
+  __syncthreads();
+  if (EPI == EPI_FWD && ep.ypart) {
+    // the head on this tile's 128 columns: two threads per row, 64 columns each
+    const int row = tid / 2, h = (tid % 2) * 64;
+    float acc_y = 0.f;
+#pragma unroll
+    for (int c = 0; c < 64; c += 8) {
+      const bf16x8 av = *reinterpret_cast<const bf16x8 *>(T + row * kEpPitch + h + c);
+      const bf16x8 wv = *reinterpret_cast<const bf16x8 *>(ep.w4 + n0 + h + c);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc_y += (float)av[t] * (float)wv[t];
+    }
+    acc_y += __shfl_xor(acc_y, 1, 64);
+    if (h == 0) ep.ypart[(size_t)bx * M + m0 + row] = acc_y;
+  }
+  tile_to_global(T, ep.out, N, m0, n0);
+  if (EPI == EPI_INF) return;
+  tile_to_global(TT, ep.outT, M, n0, m0);
+  if (EPI == EPI_FWD) {
+    // silu'(z), transposed, for the backward pass
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int nl = wn * 64 + b * 16 + lane % 16, ml = wm * 64 + a * 16 + 4 * (lane / 16);
+        bf16x4 d;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float z = r[a][b][t], sg = sigmoid(z);
+          d[t] = static_cast<bf16>(sg * (1.0f + z * (1.0f - sg)));
+        }
+        *reinterpret_cast<bf16x4 *>(TT + nl * kEpPitch + ml) = d;
+      }
+    __syncthreads();
+    tile_to_global(TT, ep.dT, M, n0, m0);
+  }
 }
 
-// Features -> padded, normalised bf16 input rows (and their transpose):
-// x_f = 2 (p_f - lo_f) / (hi_f - lo_f) - 1 for the nin features of a row.
+// Features -> padded, normalised bf16 input rows:
+// x_f = 2 (p_f - lo_f) / (hi_f - lo_f) - 1 for the nin features of a row,
+// 0 for the padding; one thread per element, features fastest (coalesced).
+__device__ __forceinline__ bf16 input_value(const double *__restrict__ P,
+                                            const int *__restrict__ idx, int rows, int nin,
+                                            const double *__restrict__ lo,
+                                            const double *__restrict__ hi, int r, int f) {
+  float v = 0.f;
+  if (r < rows && f < nin) {
+    const int src = idx ? idx[r] : r;
+    v = (float)(2.0 * (P[(size_t)src * nin + f] - lo[f]) / (hi[f] - lo[f]) - 1.0);
+  }
+  return static_cast<bf16>(v);
+}
+
+// X [Mpad x 128] (every element) and, when XT is given, its transpose
+// XT [128 x Mpad]: one thread per 8 consecutive elements (one 16-byte store).
+// `zero`, when given, is cleared (the loss accumulator of a training step).
 __global__ void prep_input_kernel(const double *__restrict__ P, const int *__restrict__ idx,
                                   int rows, int nin, const double *__restrict__ lo,
-                                  const double *__restrict__ hi, bf16 *X, bf16 *XT, int Mpad) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= Mpad) return;
-  const bool valid = r < rows;
-  const int src = valid ? (idx ? idx[r] : r) : 0;
-  for (int f = 0; f < kInPad; ++f) {
-    float v = 0.f;
-    if (valid && f < nin)
-      v = (float)(2.0 * (P[(size_t)src * nin + f] - lo[f]) / (hi[f] - lo[f]) - 1.0);
-    const bf16 b = static_cast<bf16>(v);
-    X[(size_t)r * kInPad + f] = b;
-    if (XT) XT[(size_t)f * Mpad + r] = b;
+                                  const double *__restrict__ hi, bf16 *X, bf16 *XT, int Mpad,
+                                  float *zero) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nc = (int64_t)Mpad * kInPad / 8;
+  if (zero && c == 0) *zero = 0.f;
+  bf16x8 v;
+  if (c < nc) {
+    const int r = (int)(c / (kInPad / 8)), f0 = (int)(c % (kInPad / 8)) * 8;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = input_value(P, idx, rows, nin, lo, hi, r, f0 + t);
+    *reinterpret_cast<bf16x8 *>(X + 8 * c) = v;
+  } else if (XT && c < 2 * nc) {
+    const int64_t e = 8 * (c - nc);
+    const int f = (int)(e / Mpad), r0 = (int)(e % Mpad);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = input_value(P, idx, rows, nin, lo, hi, r0 + t, f);
+    *reinterpret_cast<bf16x8 *>(XT + e) = v;
   }
 }
 
@@ -198,26 +338,32 @@ __global__ void head_forward_kernel(const bf16 *__restrict__ Aact, const bf16 *_
   if (lane == 0) y[r] = s + b[0];
 }
 
-// Loss gradient and the head's backward: dy = 2 (y - t) / B (rows >= valid
-// are padding with dy = 0); dZ4 = dy w silu'(Z4) (bf16, and transposed).
-// Head backward on a 64 x 64 tile of (rows, features): dy = 2 (y - t) / n
-// per row, dZ = dy w4 silu'(Z) (row-major) and its transpose dZT, staged in
-// LDS so that both stores are coalesced; one loss partial per tile row block.
+// Loss gradient and the head's backward on a 64 x 64 tile of (rows,
+// features): y = b4 + the partial head outputs of the last hidden layer's
+// column tiles, dy = 2 (y - t) / n per row (0 for padding rows >= valid),
+// dZ = dy w4 silu'(z) with the derivative read from dT [features x rows];
+// dZT is written along the rows, dZ along the features through an LDS tile,
+// both with 16-byte accesses.  One loss partial per tile row block.
 __global__ void __launch_bounds__(256)
-head_backward_kernel(const float *__restrict__ y, const double *__restrict__ T,
-                     const int *__restrict__ idx, int valid, int Mpad,
-                     const bf16 *__restrict__ w, const float *__restrict__ Z, float *dy,
+head_backward_kernel(const float *__restrict__ ypart, const float *__restrict__ b4,
+                     const double *__restrict__ T, const int *__restrict__ idx, int valid, int Mpad,
+                     const bf16 *__restrict__ w, const bf16 *__restrict__ dT, float *dy,
                      bf16 *dZ, bf16 *dZT, float *loss_acc, double tmean, double tstd) {
+  constexpr int kP = 72;  // tile pitch (bf16): rows of 144 bytes, 16-byte aligned
   __shared__ float gs[64];
   __shared__ float es[64];
-  __shared__ bf16 tile[64][66];
+  __shared__ __attribute__((aligned(16))) bf16 tile[64 * kP];
   const int r0 = blockIdx.y * 64, n0 = blockIdx.x * 64, tid = threadIdx.x;
   if (tid < 64) {
     const int r = r0 + tid;
     float g = 0.f, e2 = 0.f;
     if (r < valid) {
+      // the head's output: bias + the column tiles' partial sums, in order
+      float y = b4[0];
+#pragma unroll
+      for (int j = 0; j < kH / kTile; ++j) y += ypart[(size_t)j * Mpad + r];
       const float t = (float)((T[idx ? idx[r] : r] - tmean) / tstd);
-      const float e = y[r] - t;
+      const float e = y - t;
       g = 2.f * e / (float)valid;
       e2 = e * e / (float)valid;
     }
@@ -231,29 +377,50 @@ head_backward_kernel(const float *__restrict__ y, const double *__restrict__ T,
     for (int k = 0; k < 64; ++k) l += es[k];
     atomicAdd(loss_acc, l);
   }
-  // 64 x 64 elements, 16 per thread: consecutive threads along the features
-  for (int q = tid; q < 64 * 64; q += 256) {
-    const int rr = q / 64, nn = q % 64;
-    const size_t i = (size_t)(r0 + rr) * kH + n0 + nn;
-    const bf16 v = static_cast<bf16>(gs[rr] * (float)w[n0 + nn] * dsilu(Z[i]));
-    dZ[i] = v;
-    tile[nn][rr] = v;
+  // 512 chunks of 8 rows of one feature, 2 per thread
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int q = tid + 256 * i, nn = q / 8, rc = (q % 8) * 8;
+    const size_t o = (size_t)(n0 + nn) * Mpad + r0 + rc;
+    const bf16x8 d = *reinterpret_cast<const bf16x8 *>(dT + o);
+    const float wn = (float)w[n0 + nn];
+    bf16x8 v;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = static_cast<bf16>(gs[rc + t] * wn * (float)d[t]);
+    *reinterpret_cast<bf16x8 *>(dZT + o) = v;
+    *reinterpret_cast<bf16x8 *>(tile + nn * kP + rc) = v;
   }
   __syncthreads();
-  for (int q = tid; q < 64 * 64; q += 256) {
-    const int nn = q / 64, rr = q % 64;
-    dZT[(size_t)(n0 + nn) * Mpad + r0 + rr] = tile[nn][rr];
+  // 512 chunks of 8 features of one row
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int q = tid + 256 * i, rr = q / 8, nc = (q % 8) * 8;
+    bf16x8 v;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = tile[(nc + t) * kP + rr];
+    *reinterpret_cast<bf16x8 *>(dZ + (size_t)(r0 + rr) * kH + n0 + nc) = v;
   }
 }
 
 // Column reduction for the head: gw[n] = sum_r dy[r] A[r,n]; gb = sum_r dy[r].
-// Reads the transposed activations AT[n][r] (contiguous in r).
+// Reads the transposed activations AT[n][r] (contiguous in r), eight rows per
+// 16-byte load (Mpad a multiple of 128).
 __global__ void head_wgrad_kernel(const float *__restrict__ dy, const bf16 *__restrict__ AT,
                                   int Mpad, float *gw, float *gb) {
   const int n = blockIdx.x;  // 0..kH (kH -> bias)
   float s = 0.f;
-  for (int r = threadIdx.x; r < Mpad; r += blockDim.x)
-    s += dy[r] * (n < kH ? (float)AT[(size_t)n * Mpad + r] : 1.f);
+  for (int q = threadIdx.x; q < Mpad / 8; q += blockDim.x) {
+    const f32x4 d0 = *reinterpret_cast<const f32x4 *>(dy + 8 * q);
+    const f32x4 d1 = *reinterpret_cast<const f32x4 *>(dy + 8 * q + 4);
+    if (n < kH) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(AT + (size_t)n * Mpad + 8 * q);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) s += d0[t] * (float)a[t] + d1[t] * (float)a[t + 4];
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) s += d0[t] + d1[t];
+    }
+  }
   __shared__ float red[256];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -266,11 +433,32 @@ __global__ void head_wgrad_kernel(const float *__restrict__ dy, const bf16 *__re
   }
 }
 
-// Bias gradient: row sums of dZ^T [N x Mpad].
-__global__ void bias_grad_kernel(const bf16 *__restrict__ dZT, int Mpad, float *gb) {
-  const int n = blockIdx.x;
+// The weight and bias gradients of a layer, in one launch: work-groups
+// [0, nsum) sum the split-K partials, out[i] = sum over z of part[z*n + i]
+// in split order (deterministic, four elements per thread; none when the
+// GEMM wrote the gradient itself); work-group nsum + j sums row j of
+// dZ^T [kH x Mpad] into the bias gradient gb[j], eight elements per 16-byte
+// load (Mpad a multiple of 128).
+__global__ void __launch_bounds__(256)
+layer_grad_kernel(const float *__restrict__ part, int splits, int64_t n, float *__restrict__ out,
+                  int nsum, const bf16 *__restrict__ dZT, int Mpad, float *gb) {
+  if ((int)blockIdx.x < nsum) {
+    const int64_t i = 4 * (blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int z = 0; z < splits; ++z)
+      acc += *reinterpret_cast<const f32x4 *>(part + (size_t)z * n + i);
+    *reinterpret_cast<f32x4 *>(out + i) = acc;
+    return;
+  }
+  const int j = (int)blockIdx.x - nsum;
+  const bf16x8 *row = reinterpret_cast<const bf16x8 *>(dZT + (size_t)j * Mpad);
   float s = 0.f;
-  for (int r = threadIdx.x; r < Mpad; r += blockDim.x) s += (float)dZT[(size_t)n * Mpad + r];
+  for (int q = threadIdx.x; q < Mpad / 8; q += blockDim.x) {
+    const bf16x8 v = row[q];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) s += (float)v[t];
+  }
   __shared__ float red[256];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -278,22 +466,21 @@ __global__ void bias_grad_kernel(const bf16 *__restrict__ dZT, int Mpad, float *
     if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
     __syncthreads();
   }
-  if (threadIdx.x == 0) gb[n] = red[0];
+  if (threadIdx.x == 0) gb[j] = red[0];
 }
 
-// out[i] = sum over the splits of part[z*n + i], in split order (deterministic).
-__global__ void splitk_sum_kernel(const float *__restrict__ part, int splits, int64_t n,
-                                  float *__restrict__ out) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float acc = 0.f;
-  for (int z = 0; z < splits; ++z) acc += part[(size_t)z * n + i];
-  out[i] = acc;
-}
+// Adam on the flat fp32 master parameters; the new value of a weight-matrix
+// entry also goes to its bf16 copies (W and its transpose; the head's w4),
+// which the next step's GEMMs read.
+struct CastMap {
+  int64_t off[kLayers + 1];  // start of W_0..W_3 and of w4 in the flat array
+  int in[kLayers];           // columns of W_l
+  bf16 *Wb[kLayers], *WbT[kLayers], *w4b;
+};
 
-// Adam on the flat fp32 master parameters.
 __global__ void adam_kernel(float *p, const float *__restrict__ g, float *m, float *v, int64_t n,
-                            float lr, float b1, float b2, float eps, float c1, float c2) {
+                            float lr, float b1, float b2, float eps, float c1, float c2,
+                            CastMap cm) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -301,7 +488,19 @@ __global__ void adam_kernel(float *p, const float *__restrict__ g, float *m, flo
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  p[i] = pi;
+  const bf16 b = static_cast<bf16>(pi);
+#pragma unroll
+  for (int l = 0; l < kLayers; ++l) {
+    const int64_t e = i - cm.off[l];
+    if (e >= 0 && e < (int64_t)kH * cm.in[l]) {
+      const int o = (int)(e / cm.in[l]), c = (int)(e % cm.in[l]);
+      cm.Wb[l][e] = b;
+      cm.WbT[l][(size_t)c * kH + o] = b;
+    }
+  }
+  if (i >= cm.off[kLayers] && i < cm.off[kLayers] + kH) cm.w4b[i - cm.off[kLayers]] = b;
 }
 
 // bf16 copies of a weight matrix W [out x in] and its transpose.
@@ -341,9 +540,10 @@ struct fdadi_mlp {
   bf16 *Wb[kLayers] = {}, *WbT[kLayers] = {}, *w4b = nullptr;
   bf16 *X = nullptr, *XT = nullptr;
   bf16 *Act[kLayers] = {}, *ActT[kLayers] = {};
-  float *Z[kLayers] = {};
+  bf16 *dT[kLayers] = {};  // silu'(pre-activations) [kH x rows], transposed
   bf16 *dZ = nullptr, *dZT = nullptr, *dZ2 = nullptr, *dZT2 = nullptr;
   float *y = nullptr, *dy = nullptr, *loss = nullptr;
+  float *ypart = nullptr;  // training: head partial outputs [kH / 128 x rows]
   float *wpart = nullptr;  // split-K partial weight gradients [kMaxSplits x kH x kH]
   int n_in = kIn;          // features per input row
   double *lohi = nullptr;  // [lo(kMaxIn) | hi(kMaxIn)]
@@ -358,10 +558,10 @@ int in_dim(int l) { return l == 0 ? kInPad : kH; }
 
 int gemm(int epi, const bf16 *A, const bf16 *B, int M, int N, int K, const EpiArgs &ep,
          hipStream_t s, int splits = 1) {
-  FDADI_REQUIRE(M % kTile == 0 && N % kTile == 0 && K % kKStep == 0, FDADI_ERR_ARG,
-                "mlp gemm: dimensions must be multiples of 128 x 128 x 32");
-  FDADI_REQUIRE(splits >= 1 && (splits == 1 || epi == EPI_F32) && K % (kKStep * splits) == 0,
-                FDADI_ERR_ARG, "mlp gemm: split-K needs the fp32 epilogue and K % (32 s) == 0");
+  FDADI_REQUIRE(M % kTile == 0 && N % kTile == 0 && K % kKBlock == 0, FDADI_ERR_ARG,
+                "mlp gemm: dimensions must be multiples of 128 x 128 x 128");
+  FDADI_REQUIRE(splits >= 1 && (splits == 1 || epi == EPI_F32) && K % (kKBlock * splits) == 0,
+                FDADI_ERR_ARG, "mlp gemm: split-K needs the fp32 epilogue and K % (128 s) == 0");
   dim3 grid(N / kTile, M / kTile, splits);
   switch (epi) {
     case EPI_FWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
@@ -385,22 +585,31 @@ int refresh_bf16(fdadi_mlp *m, hipStream_t s) {
   return FDADI_OK;
 }
 
-// Forward over rows [0, Mpad) of the prepared input m->X; train keeps Z and
-// the transposed activations for the backward pass.
+// Forward over rows [0, Mpad) of the prepared input m->X.  Training keeps
+// the transposed activations and SiLU derivatives for the backward pass and
+// leaves the head's output as per-column-tile partial sums (m->ypart, summed
+// by the head's backward); inference computes the head into m->y.
 int forward(fdadi_mlp *m, int Mpad, bool train, hipStream_t s) {
   const bf16 *in = m->X;
   for (int l = 0; l < kLayers; ++l) {
     EpiArgs ep{};
     ep.bias = m->p + m->off_b[l];
-    ep.Z = train ? m->Z[l] : nullptr;
+    ep.dT = train ? m->dT[l] : nullptr;
     ep.out = m->Act[l];
     ep.outT = train ? m->ActT[l] : nullptr;
+    if (train && l == kLayers - 1) {
+      // the head's partial outputs come from this layer's epilogue
+      ep.w4 = m->w4b;
+      ep.ypart = m->ypart;
+    }
     FDADI_RET(gemm(train ? EPI_FWD : EPI_INF, in, m->Wb[l], Mpad, kH, in_dim(l), ep, s));
     in = m->Act[l];
   }
-  hipLaunchKernelGGL(head_forward_kernel, dim3((Mpad + 3) / 4), dim3(256), 0, s,
-                     m->Act[kLayers - 1], m->w4b, m->p + m->off_b4, Mpad, m->y);
-  FDADI_CHECK_LAUNCH();
+  if (!train) {
+    hipLaunchKernelGGL(head_forward_kernel, dim3((Mpad + 3) / 4), dim3(256), 0, s,
+                       m->Act[kLayers - 1], m->w4b, m->p + m->off_b4, Mpad, m->y);
+    FDADI_CHECK_LAUNCH();
+  }
   return FDADI_OK;
 }
 
@@ -487,10 +696,9 @@ int fdadi_mlp_predict_noalloc(const fdadi_mlp *m, const double *d_P, int64_t n, 
   // every row of X [0, Mpad) is written (padding rows as zeros), every
   // activation and head row of [0, Mpad) by the GEMMs: nothing carries over
   // from the previous call
-  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s, d_P,
-                     (const int *)nullptr, rows, m->n_in, m->lohi, m->lohi + kMaxIn, X,
-                     (bf16 *)nullptr,
-                     Mpad);
+  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for((int64_t)Mpad * kInPad / 8, 256)), dim3(256),
+                     0, s, d_P, (const int *)nullptr, rows, m->n_in, m->lohi, m->lohi + kMaxIn, X,
+                     (bf16 *)nullptr, Mpad, (float *)nullptr);
   FDADI_CHECK_LAUNCH();
   FDADI_RET(forward_inference(m, X, A0, A1, y, Mpad, s));
   hipLaunchKernelGGL(unscale_kernel, dim3(grid_for(rows, 256)), dim3(256), 0, s, y, rows,
@@ -550,7 +758,7 @@ int fdadi_mlp_create_features(int32_t n_in, int32_t max_rows, const float *init_
     al((void **)&m->WbT[l], sizeof(bf16) * kH * in_dim(l));
     al((void **)&m->Act[l], sizeof(bf16) * R * kH);
     al((void **)&m->ActT[l], sizeof(bf16) * R * kH);
-    al((void **)&m->Z[l], sizeof(float) * R * kH);
+    al((void **)&m->dT[l], sizeof(bf16) * R * kH);
   }
   al((void **)&m->w4b, sizeof(bf16) * kH);
   al((void **)&m->X, sizeof(bf16) * R * kInPad);
@@ -560,6 +768,7 @@ int fdadi_mlp_create_features(int32_t n_in, int32_t max_rows, const float *init_
   al((void **)&m->dZ2, sizeof(bf16) * R * kH);
   al((void **)&m->dZT2, sizeof(bf16) * R * kH);
   al((void **)&m->y, sizeof(float) * R);
+  al((void **)&m->ypart, sizeof(float) * (kH / kTile) * R);
   al((void **)&m->dy, sizeof(float) * R);
   al((void **)&m->loss, sizeof(float));
   al((void **)&m->wpart, sizeof(float) * kMaxSplits * kH * kH);
@@ -579,11 +788,11 @@ int fdadi_mlp_create_features(int32_t n_in, int32_t max_rows, const float *init_
     set_error(std::string("fdadi_mlp_create: ") + hipGetErrorString(e));
     // fall through to free what was allocated
     void *ptrs[] = {m->p, m->g, m->m, m->v, m->w4b, m->X, m->XT, m->dZ, m->dZT, m->dZ2,
-                    m->dZT2, m->y, m->dy, m->loss, m->wpart, m->lohi};
+                    m->dZT2, m->y, m->ypart, m->dy, m->loss, m->wpart, m->lohi};
     for (void *q : ptrs) (void)hipFree(q);
     for (int l = 0; l < kLayers; ++l) {
       (void)hipFree(m->Wb[l]); (void)hipFree(m->WbT[l]); (void)hipFree(m->Act[l]);
-      (void)hipFree(m->ActT[l]); (void)hipFree(m->Z[l]);
+      (void)hipFree(m->ActT[l]); (void)hipFree(m->dT[l]);
     }
     delete m;
     return FDADI_ERR_HIP;
@@ -595,11 +804,11 @@ int fdadi_mlp_create_features(int32_t n_in, int32_t max_rows, const float *init_
 int fdadi_mlp_destroy(fdadi_mlp *m) {
   if (!m) return FDADI_OK;
   void *ptrs[] = {m->p, m->g, m->m, m->v, m->w4b, m->X, m->XT, m->dZ, m->dZT, m->dZ2,
-                  m->dZT2, m->y, m->dy, m->loss, m->wpart, m->lohi};
+                  m->dZT2, m->y, m->ypart, m->dy, m->loss, m->wpart, m->lohi};
   for (void *q : ptrs) (void)hipFree(q);
   for (int l = 0; l < kLayers; ++l) {
     (void)hipFree(m->Wb[l]); (void)hipFree(m->WbT[l]); (void)hipFree(m->Act[l]);
-    (void)hipFree(m->ActT[l]); (void)hipFree(m->Z[l]);
+    (void)hipFree(m->ActT[l]); (void)hipFree(m->dT[l]);
   }
   delete m;
   return FDADI_OK;
@@ -633,13 +842,13 @@ static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const i
   FDADI_REQUIRE(rows > 0 && rows <= m->max_rows, FDADI_ERR_ARG,
                 "fdadi_mlp: batch rows exceed the model capacity");
   const int Mpad = (rows + kTile - 1) / kTile * kTile;
-  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s, d_P, d_idx,
-                     rows, m->n_in, m->lohi, m->lohi + kMaxIn, m->X, m->XT, Mpad);
+  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for((int64_t)Mpad * kInPad / 4, 256)), dim3(256),
+                     0, s, d_P, d_idx, rows, m->n_in, m->lohi, m->lohi + kMaxIn, m->X, m->XT, Mpad,
+                     m->loss);
   FDADI_CHECK_LAUNCH();
   FDADI_RET(forward(m, Mpad, true, s));
-  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, m->loss, (int64_t)1);
-  hipLaunchKernelGGL(head_backward_kernel, dim3(kH / 64, Mpad / 64), dim3(256), 0, s, m->y, d_T,
-                     d_idx, rows, Mpad, m->w4b, m->Z[kLayers - 1], m->dy, m->dZ, m->dZT, m->loss,
+  hipLaunchKernelGGL(head_backward_kernel, dim3(kH / 64, Mpad / 64), dim3(256), 0, s, m->ypart,
+                     m->p + m->off_b4, d_T, d_idx, rows, Mpad, m->w4b, m->dT[kLayers - 1], m->dy, m->dZ, m->dZT, m->loss,
                      m->tmean, m->tstd);
   hipLaunchKernelGGL(head_wgrad_kernel, dim3(kH + 1), dim3(256), 0, s, m->dy,
                      m->ActT[kLayers - 1], Mpad, m->g + m->off_w4, m->g + m->off_b4);
@@ -651,25 +860,21 @@ static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const i
     // (kH x in_dim) output tiles are few; the long K = Mpad is split so that
     // the grid fills the GPU, and the partial products are summed in order
     int splits = 1;
-    while (splits < kMaxSplits && Mpad % (kKStep * splits * 2) == 0 &&
+    while (splits < kMaxSplits && Mpad % (kKBlock * splits * 2) == 0 &&
            Mpad / (splits * 2) >= 512)
       splits *= 2;
     EpiArgs ew{};
     ew.C = splits > 1 ? m->wpart : m->g + m->off_W[l];
     FDADI_RET(gemm(EPI_F32, dzt, aT, kH, in_dim(l), Mpad, ew, s, splits));
-    if (splits > 1) {
-      const int64_t nw = (int64_t)kH * in_dim(l);
-      hipLaunchKernelGGL(splitk_sum_kernel, dim3(grid_for(nw, 256)), dim3(256), 0, s, m->wpart,
-                         splits, nw, m->g + m->off_W[l]);
-      FDADI_CHECK_LAUNCH();
-    }
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(kH), dim3(256), 0, s, dzt, Mpad,
-                       m->g + m->off_b[l]);
+    const int64_t nw = (int64_t)kH * in_dim(l);
+    const int nsum = splits > 1 ? (int)grid_for(nw / 4, 256) : 0;
+    hipLaunchKernelGGL(layer_grad_kernel, dim3(nsum + kH), dim3(256), 0, s, m->wpart, splits, nw,
+                       m->g + m->off_W[l], nsum, dzt, Mpad, m->g + m->off_b[l]);
     FDADI_CHECK_LAUNCH();
     if (l > 0) {
       // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1})
       EpiArgs eb{};
-      eb.Z = m->Z[l - 1];
+      eb.dT = m->dT[l - 1];
       eb.out = dz2;
       eb.outT = dzt2;
       FDADI_RET(gemm(EPI_BWD, dz, m->WbT[l], Mpad, kH, kH, eb, s));
@@ -700,10 +905,18 @@ int fdadi_mlp_train_step(fdadi_mlp *m, const double *d_P, const double *d_T, con
   m->step += 1;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+  CastMap cm;
+  for (int l = 0; l < kLayers; ++l) {
+    cm.off[l] = m->off_W[l];
+    cm.in[l] = in_dim(l);
+    cm.Wb[l] = m->Wb[l];
+    cm.WbT[l] = m->WbT[l];
+  }
+  cm.off[kLayers] = m->off_w4;
+  cm.w4b = m->w4b;
   hipLaunchKernelGGL(adam_kernel, dim3(grid_for(m->nparam, 256)), dim3(256), 0, s, m->p, m->g,
-                     m->m, m->v, m->nparam, lr, b1, b2, 1e-8f, c1, c2);
+                     m->m, m->v, m->nparam, lr, b1, b2, 1e-8f, c1, c2, cm);
   FDADI_CHECK_LAUNCH();
-  FDADI_RET(refresh_bf16(m, s));
   return FDADI_OK;
 }
 
@@ -720,10 +933,10 @@ int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
   for (int64_t r0 = 0; r0 < n; r0 += m->max_rows) {
     const int rows = (int)std::min<int64_t>(m->max_rows, n - r0);
     const int Mpad = (rows + kTile - 1) / kTile * kTile;
-    hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(Mpad, 128)), dim3(128), 0, s,
-                       d_P + r0 * m->n_in, (const int *)nullptr, rows, m->n_in, m->lohi,
-                       m->lohi + kMaxIn, m->X,
-                       (bf16 *)nullptr, Mpad);
+    hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for((int64_t)Mpad * kInPad / 8, 256)),
+                       dim3(256), 0, s, d_P + r0 * m->n_in, (const int *)nullptr, rows,
+                       m->n_in, m->lohi, m->lohi + kMaxIn, m->X, (bf16 *)nullptr, Mpad,
+                       (float *)nullptr);
     FDADI_CHECK_LAUNCH();
     FDADI_RET(forward(m, Mpad, false, s));
     hipLaunchKernelGGL(unscale_kernel, dim3(grid_for(rows, 256)), dim3(256), 0, s, m->y, rows,

@@ -297,7 +297,8 @@ int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
  * all calls on that GPU: calls that overlap on one GPU race. */
 int fdadi_mlp_predict_noalloc(const fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
                               void *stream);
-/* C (fp32) = A B^T with bf16 operands given as raw bits (test entry point). */
+/* C (fp32) = A B^T with bf16 operands given as raw bits (test entry point);
+ * M, N and K multiples of 128 (FDADI_ERR_ARG otherwise). */
 int fdadi_mlp_gemm_nt(const uint16_t *d_A, const uint16_t *d_B, int32_t M, int32_t N, int32_t K,
                       float *d_C, void *stream);
 

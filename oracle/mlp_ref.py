@@ -4,7 +4,8 @@ Same architecture and parameter layout as fd_adi_amd/csrc/mlp.hip:
 8 features (zero-padded to 128) -> 4 x [512, SiLU] -> 1, loss = mean squared
 error of the standardised target.  ``bf16=True`` rounds to bfloat16 at the
 points where the device kernels store bf16 (inputs, weights, activations,
-back-propagated pre-activation gradients); accumulation is float64 here
+SiLU derivatives at the pre-activations, back-propagated pre-activation
+gradients); accumulation is float64 here
 (float32 on the device).  With ``bf16=False`` the gradients are those of the
 float64 network, which tests/test_oracle_mlp.py pins with finite differences.
 """
@@ -90,12 +91,12 @@ def loss_and_grad(p, X, t, bf16=False):
     offs, o4, ob4, _ = layout()
     g[o4:o4 + N_H] = As[-1].T.dot(dy)
     g[ob4] = dy.sum()
-    dZ = r(dy[:, None] * r(w4)[None, :] * dsilu(Zs[-1]))
+    dZ = r(dy[:, None] * r(w4)[None, :] * r(dsilu(Zs[-1])))
     for l in range(N_LAYERS - 1, -1, -1):
         ow, ob = offs[l]
         fan_in = N_PAD if l == 0 else N_H
         g[ow:ow + N_H * fan_in] = dZ.T.dot(As[l]).ravel()
         g[ob:ob + N_H] = dZ.sum(axis=0)
         if l > 0:
-            dZ = r(dZ.dot(r(Ws[l])) * dsilu(Zs[l - 1]))
+            dZ = r(dZ.dot(r(Ws[l])) * r(dsilu(Zs[l - 1])))
     return loss, g

@@ -22,7 +22,7 @@ def test_gemm_nt_matches_numpy(fd):
     from fd_adi_amd import _lib
     from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
     rng = np.random.default_rng(0)
-    M, N, K = 256, 128, 96
+    M, N, K = 256, 128, 384
     A = rng.normal(0, 1, (M, K))
     B = rng.normal(0, 1, (N, K))
     Ab, Bb = R.bf16_bits(A), R.bf16_bits(B)

@@ -72,7 +72,8 @@ def test_device_serving_matches_host_gate(S):
     from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
     P = S.draw_params(8192, 406)
     y = S.adi_prices(P)
-    pricer, info = S.fit_surrogate(P, y, members=2, steps=300, batch=2048, max_fail=0.01)
+    # trained long enough that the gate admits part of the queries and rejects the rest
+    pricer, info = S.fit_surrogate(P, y, members=2, steps=2000, batch=2048, max_fail=0.01)
     assert pricer.device_ready()
     Q = S.draw_params(3000, 407, outside_frac=0.2)
     mean, spread = pricer.ensemble.predict(Q)
