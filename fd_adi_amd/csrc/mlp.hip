@@ -303,28 +303,43 @@ __device__ __forceinline__ bf16 input_value(const double *__restrict__ P,
 }
 
 // X [Mpad x 128] (every element) and, when XT is given, its transpose
-// XT [128 x Mpad]: one thread per 8 consecutive elements (one 16-byte store).
+// XT [128 x Mpad].  With F = 8 ceil(nin / 8) (the feature columns that can be
+// non-zero) one launch covers three ranges of threads: (row, feature < F)
+// computes the value once and stores it to X and XT; (row, 8-column chunk
+// >= F) and (8-row chunk of an XT row >= F) store zeros, 16 bytes each.
 // `zero`, when given, is cleared (the loss accumulator of a training step).
 __global__ void prep_input_kernel(const double *__restrict__ P, const int *__restrict__ idx,
                                   int rows, int nin, const double *__restrict__ lo,
                                   const double *__restrict__ hi, bf16 *X, bf16 *XT, int Mpad,
                                   float *zero) {
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t nc = (int64_t)Mpad * kInPad / 8;
-  if (zero && c == 0) *zero = 0.f;
-  bf16x8 v;
-  if (c < nc) {
-    const int r = (int)(c / (kInPad / 8)), f0 = (int)(c % (kInPad / 8)) * 8;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) v[t] = input_value(P, idx, rows, nin, lo, hi, r, f0 + t);
-    *reinterpret_cast<bf16x8 *>(X + 8 * c) = v;
-  } else if (XT && c < 2 * nc) {
-    const int64_t e = 8 * (c - nc);
-    const int f = (int)(e / Mpad), r0 = (int)(e % Mpad);
-#pragma unroll
-    for (int t = 0; t < 8; ++t) v[t] = input_value(P, idx, rows, nin, lo, hi, r0 + t, f);
-    *reinterpret_cast<bf16x8 *>(XT + e) = v;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int F = (nin + 7) / 8 * 8;
+  const int64_t n1 = (int64_t)Mpad * F, n2 = (int64_t)Mpad * (kInPad - F) / 8;
+  if (zero && t == 0) *zero = 0.f;
+  if (t < n1) {
+    const int f = (int)(t / Mpad), r = (int)(t % Mpad);
+    const bf16 v = input_value(P, idx, rows, nin, lo, hi, r, f);
+    X[(size_t)r * kInPad + f] = v;
+    if (XT) XT[(size_t)f * Mpad + r] = v;
+    return;
   }
+  const bf16x8 z = {};
+  if (t < n1 + n2) {
+    const int64_t u = t - n1;
+    const int per = (kInPad - F) / 8;
+    *reinterpret_cast<bf16x8 *>(X + (u / per) * kInPad + F + (u % per) * 8) = z;
+    return;
+  }
+  const int64_t u = t - n1 - n2;
+  if (XT && u < (int64_t)(kInPad - F) * Mpad / 8)
+    *reinterpret_cast<bf16x8 *>(XT + (size_t)F * Mpad + 8 * u) = z;
+}
+
+// Threads of prep_input_kernel for Mpad rows (with the transpose or without).
+inline int64_t prep_threads(int Mpad, int nin, bool transpose) {
+  const int F = (nin + 7) / 8 * 8;
+  return (int64_t)Mpad * F + (int64_t)Mpad * (kInPad - F) / 8 +
+         (transpose ? (int64_t)(kInPad - F) * Mpad / 8 : 0);
 }
 
 // Head: y[r] = sum_n A[r,n] w[n] + b  (one wave per row, fp32).
@@ -472,9 +487,10 @@ layer_grad_kernel(const float *__restrict__ part, int splits, int64_t n, float *
 // Adam on the flat fp32 master parameters; the new value of a weight-matrix
 // entry also goes to its bf16 copies (W and its transpose; the head's w4),
 // which the next step's GEMMs read.
+static_assert(kInPad == (1 << 7) && kH == (1 << 9), "adam_kernel indexes W_l by shifts");
 struct CastMap {
   int64_t off[kLayers + 1];  // start of W_0..W_3 and of w4 in the flat array
-  int in[kLayers];           // columns of W_l
+  int lg_in[kLayers];        // log2 of the columns of W_l (128 or 512)
   bf16 *Wb[kLayers], *WbT[kLayers], *w4b;
 };
 
@@ -494,8 +510,8 @@ __global__ void adam_kernel(float *p, const float *__restrict__ g, float *m, flo
 #pragma unroll
   for (int l = 0; l < kLayers; ++l) {
     const int64_t e = i - cm.off[l];
-    if (e >= 0 && e < (int64_t)kH * cm.in[l]) {
-      const int o = (int)(e / cm.in[l]), c = (int)(e % cm.in[l]);
+    if (e >= 0 && e < ((int64_t)kH << cm.lg_in[l])) {
+      const int o = (int)e >> cm.lg_in[l], c = (int)e & ((1 << cm.lg_in[l]) - 1);
       cm.Wb[l][e] = b;
       cm.WbT[l][(size_t)c * kH + o] = b;
     }
@@ -696,8 +712,8 @@ int fdadi_mlp_predict_noalloc(const fdadi_mlp *m, const double *d_P, int64_t n, 
   // every row of X [0, Mpad) is written (padding rows as zeros), every
   // activation and head row of [0, Mpad) by the GEMMs: nothing carries over
   // from the previous call
-  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for((int64_t)Mpad * kInPad / 8, 256)), dim3(256),
-                     0, s, d_P, (const int *)nullptr, rows, m->n_in, m->lohi, m->lohi + kMaxIn, X,
+  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(prep_threads(Mpad, m->n_in, false), 256)),
+                     dim3(256), 0, s, d_P, (const int *)nullptr, rows, m->n_in, m->lohi, m->lohi + kMaxIn, X,
                      (bf16 *)nullptr, Mpad, (float *)nullptr);
   FDADI_CHECK_LAUNCH();
   FDADI_RET(forward_inference(m, X, A0, A1, y, Mpad, s));
@@ -842,8 +858,8 @@ static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const i
   FDADI_REQUIRE(rows > 0 && rows <= m->max_rows, FDADI_ERR_ARG,
                 "fdadi_mlp: batch rows exceed the model capacity");
   const int Mpad = (rows + kTile - 1) / kTile * kTile;
-  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for((int64_t)Mpad * kInPad / 4, 256)), dim3(256),
-                     0, s, d_P, d_idx, rows, m->n_in, m->lohi, m->lohi + kMaxIn, m->X, m->XT, Mpad,
+  hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(prep_threads(Mpad, m->n_in, true), 256)),
+                     dim3(256), 0, s, d_P, d_idx, rows, m->n_in, m->lohi, m->lohi + kMaxIn, m->X, m->XT, Mpad,
                      m->loss);
   FDADI_CHECK_LAUNCH();
   FDADI_RET(forward(m, Mpad, true, s));
@@ -908,7 +924,7 @@ int fdadi_mlp_train_step(fdadi_mlp *m, const double *d_P, const double *d_T, con
   CastMap cm;
   for (int l = 0; l < kLayers; ++l) {
     cm.off[l] = m->off_W[l];
-    cm.in[l] = in_dim(l);
+    cm.lg_in[l] = in_dim(l) == kInPad ? 7 : 9;
     cm.Wb[l] = m->Wb[l];
     cm.WbT[l] = m->WbT[l];
   }
@@ -933,7 +949,7 @@ int fdadi_mlp_predict(fdadi_mlp *m, const double *d_P, int64_t n, double *d_out,
   for (int64_t r0 = 0; r0 < n; r0 += m->max_rows) {
     const int rows = (int)std::min<int64_t>(m->max_rows, n - r0);
     const int Mpad = (rows + kTile - 1) / kTile * kTile;
-    hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for((int64_t)Mpad * kInPad / 8, 256)),
+    hipLaunchKernelGGL(prep_input_kernel, dim3(grid_for(prep_threads(Mpad, m->n_in, false), 256)),
                        dim3(256), 0, s, d_P + r0 * m->n_in, (const int *)nullptr, rows,
                        m->n_in, m->lohi, m->lohi + kMaxIn, m->X, (bf16 *)nullptr, Mpad,
                        (float *)nullptr);

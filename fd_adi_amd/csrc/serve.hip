@@ -56,6 +56,8 @@ struct Box {
   double lo[8], hi[8];
 };
 
+constexpr int kGateFeatures = 13;  // columns of the error model's feature rows
+
 __global__ void serve_combine_kernel(const double *__restrict__ P, int64_t n,
                                      const double *__restrict__ Y, int M,
                                      const double *__restrict__ base, Box box, double feller_min,
@@ -64,8 +66,13 @@ __global__ void serve_combine_kernel(const double *__restrict__ P, int64_t n,
   const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= n) return;
   const double *p = P + 8 * r;
-  double sum = 0.0;
-  for (int m = 0; m < M; ++m) sum += Y[m * n + r];
+  double sum = 0.0, ymin = Y[r], ymax = Y[r];
+  for (int m = 0; m < M; ++m) {
+    const double v = Y[m * n + r];
+    sum += v;
+    ymin = fmin(ymin, v);
+    ymax = fmax(ymax, v);
+  }
   const double mean = sum / M;
   double ss = 0.0;
   for (int m = 0; m < M; ++m) {
@@ -75,7 +82,7 @@ __global__ void serve_combine_kernel(const double *__restrict__ P, int64_t n,
   const double spread = M > 1 ? sqrt(ss / (M - 1)) : 0.0;
   price[r] = base[r] + mean;
   const double feller = 2.0 * p[0] * p[1] / (p[2] * p[2]);
-  double *x = X + 10 * r;
+  double *x = X + kGateFeatures * r;
   x[0] = log(p[0]);
   x[1] = log(p[1]);
   x[2] = log(p[2]);
@@ -86,6 +93,9 @@ __global__ void serve_combine_kernel(const double *__restrict__ P, int64_t n,
   x[7] = log(p[7]);
   x[8] = log(feller);
   x[9] = log10(spread + 1e-12);
+  x[10] = mean;
+  x[11] = log10(fabs(mean) + 1e-9);
+  x[12] = log10(ymax - ymin + 1e-12);
   bool in = feller >= feller_min;
   for (int k = 0; k < 8; ++k) in = in && p[k] >= box.lo[k] && p[k] <= box.hi[k];
   inside[r] = in ? 1 : 0;

@@ -10,6 +10,7 @@ the training box, and with the FP64 batched ADI engine elsewhere.
 Features, in order: kappa, theta, sigma, rho, v0, r, T, K/S0 (S0 = 100).
 """
 
+import collections
 import ctypes
 
 import numpy as np
@@ -301,6 +302,15 @@ def cf_price(P):
 BASELINES = {"cf": cf_price, "bound": lower_bound}
 
 
+EnsembleStats = collections.namedtuple("EnsembleStats", "mean spread corr range")
+EnsembleStats.__doc__ = """Per-row statistics of an ensemble's predictions (``Ensemble.stats``)."""
+
+
+def _rows(st, sel):
+    """The rows ``sel`` of an ``EnsembleStats``."""
+    return EnsembleStats(*[np.asarray(a)[sel] for a in st])
+
+
 class Ensemble(object):
     """Independently initialised networks; the spread of their predictions
     is the per-query error estimate.  Prices are baseline + network, with the
@@ -312,9 +322,18 @@ class Ensemble(object):
         self.base = base
 
     def predict(self, P_host):
+        st = self.stats(P_host)
+        return st.mean, st.spread
+
+    def stats(self, P_host):
+        """``EnsembleStats`` of feature rows: the price (baseline + the
+        members' mean correction), the members' standard deviation (M - 1),
+        their mean correction and their range (max - min)."""
         Y = np.stack([m.predict(P_host) for m in self.members])
         spread = Y.std(axis=0, ddof=1) if len(self.members) > 1 else 0 * Y[0]
-        return BASELINES[self.base](P_host) + Y.mean(axis=0), spread
+        corr = Y.mean(axis=0)
+        return EnsembleStats(BASELINES[self.base](P_host) + corr, spread, corr,
+                             Y.max(axis=0) - Y.min(axis=0))
 
     def correction(self, P_host):
         """The members' mean without the baseline (a residual stage)."""
@@ -710,18 +729,25 @@ def calibrate_gate(ensemble, P, y, atol=1e-4, feller_min=0.0, max_fail=0.0):
 class ErrorModel(object):
     """Per-query estimate of the surrogate's error, fitted on held-out FP64
     ADI labels: a regression of log10 |surrogate - ADI| on the 8 features,
-    the log Feller ratio and the log ensemble spread.  The members' spread
-    alone cannot see the error they share; this model learns where it lies
-    (low Feller ratio, large vol-of-vol, long maturities near the money).
+    the log Feller ratio and four statistics of the ensemble's corrections
+    (log spread, mean, log |mean|, log range).  The members' spread alone
+    cannot see the error they share; this model learns where it lies (low
+    Feller ratio, large vol-of-vol, long maturities near the money, large
+    corrections).
 
     ``kind`` "trees" (default): gradient-boosted regression trees
-    (scikit-learn's histogram GBM, on the host); "mlp": a network of the
-    library's own MFMA kernels on 10 input features.  Measured on 2^20
-    labels, the trees admit 43% of held-out queries at no failure beyond
-    1e-4, the network 2% (and lets failures through): the regions where the
-    error jumps have sharp edges in the parameters, which trees follow."""
+    (scikit-learn's histogram GBM; evaluated on the device by ``Forest`` when
+    serving); "mlp": a network of the library's own MFMA kernels on the same
+    features.  The trees do far better than the network at the same
+    calibration (the regions where the error jumps have sharp edges in the
+    parameters, which trees follow).  On the held-out rows of 2^20 labels and
+    8 members, adding the mean correction, its magnitude and the range to the
+    spread raised the admission at no calibration failure from 70% to 75%
+    (mean of the two halves of the calibration rows, each calibrating the
+    other)."""
 
     FLOOR = 1e-9
+    NFEAT = 13
 
     def __init__(self, kind="trees"):
         if kind not in ("trees", "mlp"):
@@ -730,13 +756,18 @@ class ErrorModel(object):
         self.model = None
 
     @staticmethod
-    def features(P, spread):
+    def features(P, st):
+        """Feature rows [n, 13] of rows P and their ``EnsembleStats`` st (the
+        columns fdadi_serve_combine writes on the device)."""
         P = np.asarray(P, dtype=float)
+        corr = np.asarray(st.corr, dtype=float)
         return np.column_stack([log_features(P), np.log(feller_ratio(P)),
-                                np.log10(np.asarray(spread, dtype=float) + 1e-12)])
+                                np.log10(np.asarray(st.spread, dtype=float) + 1e-12), corr,
+                                np.log10(np.abs(corr) + 1e-9),
+                                np.log10(np.asarray(st.range, dtype=float) + 1e-12)])
 
-    def fit(self, P, spread, err, steps=4000, batch=8192, lr=1e-3, seed=0):
-        X = self.features(P, spread)
+    def fit(self, P, st, err, steps=4000, batch=8192, lr=1e-3, seed=0):
+        X = self.features(P, st)
         t = np.log10(np.asarray(err, dtype=float) + self.FLOOR)
         if self.kind == "trees":
             from sklearn.ensemble import HistGradientBoostingRegressor
@@ -750,14 +781,14 @@ class ErrorModel(object):
                          max_rows=batch)
         return self.model.fit(X, t, steps, batch=batch, lr=lr, seed=seed, schedule="cosine")
 
-    def __call__(self, P, spread):
-        return self.model.predict(self.features(P, spread))
+    def __call__(self, P, st):
+        return self.model.predict(self.features(P, st))
 
     def forest(self):
         """The fitted trees on the device (``Forest``; kind "trees" only)."""
         if self.kind != "trees":
             raise NotImplementedError("ErrorModel.forest: the trees model only")
-        return Forest(self.model, 10)
+        return Forest(self.model, self.NFEAT)
 
 
 def fit_surrogate(P, y, members=4, steps=16000, batch=8192, fmap="identity", atol=1e-4,
@@ -786,13 +817,13 @@ def fit_surrogate(P, y, members=4, steps=16000, batch=8192, fmap="identity", ato
     ens, hist = train_ensemble(P[tr], y[tr], members=members, steps=steps, batch=batch,
                                schedule="cosine", fmap=fmap, seed=seed)
     H = np.arange(ntr, n)
-    mean, spread = ens.predict(P[H])
-    err = np.abs(mean - y[H])
+    st = ens.stats(P[H])
+    err = np.abs(st.mean - y[H])
     nfit = int(round((1.0 - calib) * len(H)))
     em = ErrorModel(error_kind)
-    em.fit(P[H[:nfit]], spread[:nfit], err[:nfit], steps=error_steps, seed=seed + 1)
+    em.fit(P[H[:nfit]], _rows(st, slice(0, nfit)), err[:nfit], steps=error_steps, seed=seed + 1)
     Hc = H[nfit:]
-    score = em(P[Hc], spread[nfit:])
+    score = em(P[Hc], _rows(st, slice(nfit, None)))
     region = feller_ratio(P[Hc]) >= feller_min
     catol = calib_margin * atol
     tau, frac = calibrate_threshold(score[region], err[nfit:][region], catol, max_fail)
@@ -816,9 +847,9 @@ class SurrogatePricer(object):
     mean) when it lies inside the training box, its Feller ratio is at least
     ``feller_min`` and its error estimate is at most ``tau``; every other
     query goes to ``adi`` (default: ``adi_prices`` on the label meshes).  The
-    error estimate is the ensemble spread, or ``error_model(P, spread)`` when
-    given (an ``ErrorModel``, or any callable returning one estimate per
-    row)."""
+    error estimate is the ensemble spread, or ``error_model(P, stats)`` when
+    given (an ``ErrorModel``, or any callable of the rows and their
+    ``EnsembleStats`` returning one estimate per row)."""
 
     def __init__(self, ensemble, tau, lo=BOX_LO, hi=BOX_HI, feller_min=0.0, error_model=None,
                  adi=None):
@@ -831,8 +862,9 @@ class SurrogatePricer(object):
         self.error_model = error_model
         self.adi = adi if adi is not None else adi_prices
 
-    def score(self, P, spread):
-        return spread if self.error_model is None else self.error_model(P, spread)
+    def score(self, P, st):
+        """Error estimate of rows P from their ``EnsembleStats`` st."""
+        return st.spread if self.error_model is None else self.error_model(P, st)
 
     def gate(self, P, score):
         inside = np.all((P >= self.lo) & (P <= self.hi), axis=1)
@@ -851,9 +883,9 @@ class SurrogatePricer(object):
             out, ok = SizedArrayPtr().alloc(n), DeviceInts(np.zeros(n, dtype=np.int32))
             self.price_device(Pd, n, out, ok, **adi_kw)
             return out.to_numpy()[:n], ok.to_numpy()[:n].astype(bool)
-        mean, spread = self.ensemble.predict(P)
-        ok = self.gate(P, self.score(P, spread))
-        out = mean.copy()
+        st = self.ensemble.stats(P)
+        ok = self.gate(P, self.score(P, st))
+        out = st.mean.copy()
         self.last_fallback = int((~ok).sum())
         if (~ok).any():
             out[~ok] = self.adi(P[~ok], **adi_kw)
@@ -885,7 +917,8 @@ class SurrogatePricer(object):
             self._forest = self.error_model.forest()
         ws, M, s = self._ws, len(self.ensemble.members), _lib.default_stream()
         Y, base = ws.get("Y", M * n), ws.get("base", n)
-        rows, X, score = ws.get("rows", 9 * n), ws.get("X", 10 * n), ws.get("score", n)
+        rows, X = ws.get("rows", 9 * n), ws.get("X", ErrorModel.NFEAT * n)
+        score = ws.get("score", n)
         inside, idx = ws.get("inside", (n + 1) // 2), ws.get("idx", (n + 1) // 2)
         work = ws.get("work", (n + 255) // 256 + 2)
         for k, m in enumerate(self.ensemble.members):

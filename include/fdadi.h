@@ -342,9 +342,11 @@ int fdadi_forest_destroy(fdadi_forest *f);
 int fdadi_forest_predict(const fdadi_forest *f, const double *d_X, int64_t n, double *d_out,
                          void *stream);
 /* price = d_base + mean of the M member corrections d_Y (M x n, member-major);
- * d_X (n x 10) = the error model's features (log kappa, log theta, log sigma,
+ * d_X (n x 13) = the error model's features (log kappa, log theta, log sigma,
  * rho, log v0, r, log T, log K/S0, log Feller ratio 2 kappa theta / sigma^2,
- * log10(spread + 1e-12), spread the members' standard deviation with M - 1);
+ * log10(spread + 1e-12), mean, log10(|mean| + 1e-9), log10(max - min + 1e-12);
+ * spread is the members' standard deviation with M - 1, mean, max and min
+ * are over the members' corrections);
  * d_inside = row within the box [lo, hi] (host arrays of 8) with Feller ratio
  * >= feller_min. */
 int fdadi_serve_combine(const double *d_P, int64_t n, const double *d_Y, int32_t M,

@@ -145,8 +145,8 @@ def test_fitted_surrogate_gate_and_routing(S):
     print(info)
     assert info["error_model"] == "trees" and info["calib_rows"] > 600
     Q = S.draw_params(512, 307, outside_frac=0.2)
-    mean, spread = pricer.ensemble.predict(Q)
-    score = pricer.error_model(Q, spread)
+    st = pricer.ensemble.stats(Q)
+    mean, score = st.mean, pricer.error_model(Q, st)
     price, accepted = pricer.price(Q)
     inside = np.all((Q >= S.BOX_LO) & (Q <= S.BOX_HI), axis=1)
     npt.assert_array_equal(accepted, inside & (score <= pricer.tau))
