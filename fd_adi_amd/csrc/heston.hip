@@ -376,7 +376,10 @@ __device__ __forceinline__ Raw2 load_raw2(const double *__restrict__ f, unsigned
 
 // Value of the lane before / after / two before within the tile's lanes
 // (quad_perm for four columns; row_shr / row_shl within a DPP row of 16
-// lanes for 8 and 16; the tile's first and last lanes use their edge load).
+// lanes for 8 and 16; wave_shr / wave_shl across the two DPP rows of a
+// 32-column tile; the tile's first and last lanes use their edge load).  Two
+// before serves the last column only, which sits on the tile's last lane
+// (lane 31 of a 32-column tile, nv >= 32), so row_shr:2 stays in its row.
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double x) {
   const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
@@ -385,11 +388,13 @@ __device__ __forceinline__ double dpp_mov(double x) {
 }
 template <int NL>
 __device__ __forceinline__ double from_prev_col(double x) {
-  return dpp_mov<NL == 4 ? 0x90 : 0x111>(x);  // quad_perm [0,0,1,2] / row_shr:1
+  // quad_perm [0,0,1,2] / row_shr:1 / wave_shr:1
+  return dpp_mov<NL == 4 ? 0x90 : (NL == 32 ? 0x138 : 0x111)>(x);
 }
 template <int NL>
 __device__ __forceinline__ double from_next_col(double x) {
-  return dpp_mov<NL == 4 ? 0xF9 : 0x101>(x);  // quad_perm [1,2,3,3] / row_shl:1
+  // quad_perm [1,2,3,3] / row_shl:1 / wave_shl:1
+  return dpp_mov<NL == 4 ? 0xF9 : (NL == 32 ? 0x130 : 0x101)>(x);
 }
 template <int NL>
 __device__ __forceinline__ double from_prev2_col(double x) {
@@ -563,7 +568,8 @@ template <int MODE, int THREADS, int NL, int MAXL>
 __global__ void __launch_bounds__(THREADS)
 spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restrict__ U,
                   const double *__restrict__ Y, double *__restrict__ out) {
-  static_assert(NL == 4 || NL == 8 || NL == 16, "spot tiles are 4, 8 or 16 columns wide");
+  static_assert(NL == 4 || NL == 8 || NL == 16 || NL == 32,
+                "spot tiles are 4, 8, 16 or 32 columns wide");
   extern __shared__ double lds[];
   constexpr int TPL = THREADS / NL;
   constexpr bool HAS_Y = MODE == SM_HV || MODE == SM_CS || MODE == SM_MCS;
@@ -1337,7 +1343,7 @@ struct fdadi_hb {
   int var_tpl = 1, var_rows = 256;  // variance sweep: lanes per line, rows per 256-thread tile
   std::vector<int> order;           // options, largest first
   int spot_maxl = 8;  // rows per thread in the spot sweep (8, or 16 for long lines)
-  int spot_nl = 4;    // columns per spot-sweep tile (4, 8 or 16)
+  int spot_nl = 4;    // columns per spot-sweep tile (4, 8, 16 or 32)
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
   bool dev_scalars = false;        // last configure took theta and dt from device memory
@@ -1376,12 +1382,12 @@ Ctx ctx_of(const fdadi_hb *hb) {
   } while (0)
 
 // Spot-sweep shape for lines of length n: `threads` per work-group (0 =
-// default), `nl` columns per tile (4, 8 or 16), and the rows per thread (8,
-// or 16 for long lines).  A thread keeps four values per row in registers, so
-// a work-group has at most 512 threads (two waves per SIMD).
+// default), `nl` columns per tile (4, 8, 16 or 32), and the rows per thread
+// (8, or 16 for long lines).  A thread keeps four values per row in
+// registers, so a work-group has at most 512 threads (two waves per SIMD).
 int spot_shape(int64_t n, int threads, int nl, int *t_out, int *maxl_out) {
-  if (nl != 4 && nl != 8 && nl != 16) {
-    set_error("fdadi_hb_set_shape: spot tiles are 4, 8 or 16 columns wide");
+  if (nl != 4 && nl != 8 && nl != 16 && nl != 32) {
+    set_error("fdadi_hb_set_shape: spot tiles are 4, 8, 16 or 32 columns wide");
     return FDADI_ERR_ARG;
   }
   if (threads == 0) {
@@ -1397,7 +1403,7 @@ int spot_shape(int64_t n, int threads, int nl, int *t_out, int *maxl_out) {
   }
   const int tpl = threads / nl;
   const bool t_ok = threads >= 64 && threads <= 512 && (threads & (threads - 1)) == 0 &&
-                    tpl >= 1 && (nl == 4 || threads >= 256);
+                    tpl >= 1 && (nl == 4 || threads >= 256) && (nl != 32 || threads == 512);
   if (t_ok && (int64_t)tpl * kMAXLs >= n) {
     *t_out = threads;
     *maxl_out = kMAXLs;
@@ -1492,19 +1498,24 @@ template <int MODE, int NL>
 int launch_spot_wide(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                      hipStream_t s) {
   const bool long_rows = hb->spot_maxl == 2 * kMAXLs;
+  if constexpr (NL != 32) {  // 32-column tiles: 512 threads (16 lanes per line)
+    switch (hb->spot_threads * 2 + (long_rows ? 1 : 0)) {
+      case 512: return launch_spot_t<MODE, 256, NL, kMAXLs>(hb, U, Y, out, s);
+      case 513: return launch_spot_t<MODE, 256, NL, 2 * kMAXLs>(hb, U, Y, out, s);
+    }
+  }
   switch (hb->spot_threads * 2 + (long_rows ? 1 : 0)) {
-    case 512: return launch_spot_t<MODE, 256, NL, kMAXLs>(hb, U, Y, out, s);
-    case 513: return launch_spot_t<MODE, 256, NL, 2 * kMAXLs>(hb, U, Y, out, s);
     case 1024: return launch_spot_t<MODE, 512, NL, kMAXLs>(hb, U, Y, out, s);
     case 1025: return launch_spot_t<MODE, 512, NL, 2 * kMAXLs>(hb, U, Y, out, s);
   }
-  set_error("spot sweep: unsupported shape for 8- or 16-column tiles");
+  set_error("spot sweep: unsupported shape for 8-, 16- or 32-column tiles");
   return FDADI_ERR_UNSUPPORTED;
 }
 
 template <int MODE>
 int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *out,
                 hipStream_t s) {
+  if (hb->spot_nl == 32) return launch_spot_wide<MODE, 32>(hb, U, Y, out, s);
   if (hb->spot_nl == 16) return launch_spot_wide<MODE, 16>(hb, U, Y, out, s);
   if (hb->spot_nl == 8) return launch_spot_wide<MODE, 8>(hb, U, Y, out, s);
   if (hb->spot_maxl == 2 * kMAXLs) {
@@ -2031,6 +2042,12 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
     const int nl = lines == 0 ? 4 : lines;
     int t = 0, ml = 0;
     FDADI_RET(spot_shape(hb->nSmax, threads, nl, &t, &ml));
+    if (nl == 32) {
+      // the last column must fall on a tile's last lane (see from_prev2_col)
+      for (const HOpt &o : hb->h_opts)
+        FDADI_REQUIRE(o.nv >= 32, FDADI_ERR_ARG,
+                      "fdadi_hb_set_shape: 32-column spot tiles need at least 32 variance nodes");
+    }
     hb->spot_threads = t;
     hb->spot_maxl = ml;
     if (nl != hb->spot_nl) {

@@ -244,8 +244,9 @@ int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream);
 int fdadi_hb_replay(fdadi_hb *hb, void *stream);
 /* Launch shape per sweep kind, used by the autotuner (0 = library default):
  * kind 0 (spot sweep): `threads` per work-group (64..512) and `lines` =
- * variance columns per tile (4, 8 or 16; 0 = 4; 8 and 16 need 256 or 512
- * threads);
+ * variance columns per tile (4, 8, 16 or 32; 0 = 4; 8 and 16 need 256 or 512
+ * threads, 32 needs 512 threads and at least 32 variance nodes in every
+ * option);
  * kind 1 (variance sweep): `threads` = lanes per variance line (1..64, the
  * work-group has 256 threads), `lines` must be 0.  get_shape reports the
  * columns (kind 0) / spot rows (kind 1) per tile.  Discards a captured graph. */

@@ -84,10 +84,10 @@ def test_graph_replay_equals_eager(fd):
 
 
 def test_launch_shapes_agree(fd):
-    P, S, Vs, dts, O = draw(3, 3, nS_range=(40, 100), nv_range=(20, 60))
+    P, S, Vs, dts, O = draw(3, 3, nS_range=(40, 100), nv_range=(32, 64))
     outs = []
     for ts, nl, tv in ((0, 0, 0), (256, 4, 16), (512, 4, 64), (256, 8, 16), (512, 8, 8),
-                       (256, 16, 8), (512, 16, 32)):
+                       (256, 16, 8), (512, 16, 32), (512, 32, 16)):
         hb = fd.HestonBatch(P, S, Vs, dts)
         if ts:
             hb.set_shape(0, ts, nl)
@@ -139,6 +139,47 @@ def test_every_column_residue_every_shape_matches_oracle(fd):
             tol, floor = grid_tolerance(want[k], alts[k])
             err = float(np.max(np.abs(got[k] - want[k])))
             assert err <= tol, (shape, k, got[k].shape, err, tol)
+
+
+def test_32_column_tiles_every_residue_match_oracle(fd):
+    """32-column spot tiles (neighbour columns across the tile's two DPP rows
+    by wave shifts): variance sizes 32..65 cover every residue modulo 32, so
+    the last tile overlaps the one before it; Craig-Sneyd and
+    Hundsdorfer-Verwer, 10 steps, against the oracle.  Fewer than 32
+    variance nodes are refused."""
+    rng = np.random.default_rng(12)
+    P, S, Vs, dts, O = [], [], [], [], []
+    for nv in range(32, 66):
+        nS = int(rng.integers(20, 44))
+        K, theta = 100 * rng.uniform(0.8, 1.2), rng.uniform(0.02, 0.1)
+        p = dict(r=0.03, kappa=2.0, theta=theta, sigma=0.4, rho=-0.6)
+        spots = sinh_space(K, 5 * K, 7.0, nS)
+        vars_ = exponential_space(0, theta, 5 * theta, 4.0, nv)
+        P.append(p)
+        S.append(spots)
+        Vs.append(vars_)
+        dts.append(1.0 / 10)
+        O.append(HestonOracle(HestonParams(spot=100, strike=K, r=p["r"], v0=theta, T=1.0,
+                                           kappa=p["kappa"], theta=theta, sigma=p["sigma"],
+                                           rho=p["rho"]), spots, vars_))
+    for scheme, solve in (("cs", "solve_craigsneyd"), ("hv", "solve_hundsdorferverwer")):
+        want = [getattr(o, solve)(10, dts[k], o.payoff(), theta=0.5) for k, o in enumerate(O)]
+        alts = [[getattr(a, solve)(10, dts[k], o.payoff(), theta=0.5) for a in alternates(o)]
+                for k, o in enumerate(O)]
+        hb = fd.HestonBatch(P, S, Vs, dts)
+        hb.set_shape(0, 512, 32)
+        assert hb.get_shape(0) == (512, 32)
+        hb.set_state([o.payoff() for o in O])
+        hb.configure(scheme, 0.5)
+        hb.step(10)
+        got = hb.get_state()
+        for k in range(len(O)):
+            tol, floor = grid_tolerance(want[k], alts[k])
+            err = float(np.max(np.abs(got[k] - want[k])))
+            assert err <= tol, (scheme, k, got[k].shape, err, tol)
+    small = fd.HestonBatch(P[:1], [S[0]], [Vs[0][:31]], dts[:1])
+    with pytest.raises(Exception):
+        small.set_shape(0, 512, 32)
 
 
 def test_full_size_config1_steps(fd):
