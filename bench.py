@@ -60,38 +60,49 @@ def config1():
     return single(2048, 1024, 2000, -0.7)
 
 
-def run_single(opt, scheme, theta, K, W, smoothing=0, graph=False):
+def run_single(opt, scheme, theta, K, W, smoothing=0, graph=False, tune=False):
     """Time K solves of one option (state reset from a device-resident copy).
 
     Every solve rebuilds the factorisations (configure) and runs all nt
-    steps.  With ``graph`` the steps of a solve are one HIP graph replay
-    (captured once, untimed); the last timed solve launches its stages one by
-    one with a HIP event pair around each (the per-kernel durations)."""
+    steps.  With ``graph`` a solve is one HIP graph replay (captured once,
+    untimed: smoothing steps, both refactorisations and the scheme's steps),
+    and the per-stage durations come from one eager solve before the capture;
+    without, the last timed solve launches its stages one by one with a HIP
+    event pair around each.  ``tune``: the launch shapes are autotuned for
+    the scheme first (untimed)."""
     import fd_adi_amd
     hb = fd_adi_amd.HestonBatch([opt.params], [opt.spots], [opt.vars], [opt.dt])
     hb.set_state([opt.payoff()])
+    if tune:
+        hb.configure(scheme, theta, 1.0)
+        log("single %dx%d: autotuned launch shapes %s" % (opt.spots.size, opt.vars.size,
+                                                         hb.autotune()))
     init = hb.state_view(0).copy(True)          # payoff kept in HBM
     view = hb.state_view(0)
     n = opt.nt - smoothing
-    if graph:
-        assert not smoothing
-        hb.configure(scheme, theta, 1.0)
-        hb.capture(n)
 
-    def solve(timed=False):
+    def eager(timed=False):
         view.copy_from(init)
-        stages = None
         if smoothing:
             hb.configure("implicit", 1.0, 0.5)
             hb.step(2 * smoothing)
         hb.configure(scheme, theta, 1.0)
         if timed:
-            stages = hb.step_timed(n)
-        elif graph:
+            return hb.step_timed(n)
+        hb.step(n)
+        return None
+
+    graph_stages = None
+    if graph:
+        graph_stages = eager(timed=True)
+        hb.capture_solve(scheme, opt.nt, theta=theta, smoothing_steps=smoothing)
+
+    def solve(timed=False):
+        if graph:
+            view.copy_from(init)
             hb.replay()
-        else:
-            hb.step(n)
-        return stages
+            return None
+        return eager(timed)
 
     for w in range(W):
         solve(timed=(w == 0))  # also creates the event pool outside the timed region
@@ -105,7 +116,7 @@ def run_single(opt, scheme, theta, K, W, smoothing=0, graph=False):
     hb.sync()
     wall = time.perf_counter() - t0
     price = hb.get_state()[0][opt.idx]
-    return wall, stages, price
+    return wall, (graph_stages if graph else stages), price
 
 
 def batched(n_opt, seed, K, W):
@@ -449,10 +460,10 @@ def main():
     if not args.no_extra:
         from fd_adi_amd.workloads import single
         o0 = single(150, 150, 150, 0.0, spot_max=500.0, var_max=0.2)
-        w0, _, p0 = run_single(o0, "douglas", 0.5, 20, 2, smoothing=2)
+        w0, _, p0 = run_single(o0, "douglas", 0.5, 20, 2, smoothing=2, graph=True)
         extra["plain_150x150_douglas_smoothed"] = {
             "options_per_s": round(20 / w0, 2), "price": p0,
-            "launches_per_option": 2 * 4 + 2 * 148}
+            "launches_per_option": 2 * 4 + 2 * 148 + 6, "graph": "one replay per solve"}
         loops = single_loops(2, 16)
         extra["single_option_loop_cs_options_per_s"] = loops
         rate, sec, npts, shapes, cfg_s = batched(4096, 2024, 3, 1)

@@ -107,6 +107,7 @@ _SIGS = {
     "fdadi_hb_step_timed": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i32, _P(c_i32)]),
     "fdadi_hb_capture": (c_int, [c_vp, c_i32, c_vp]),
     "fdadi_hb_replay": (c_int, [c_vp, c_vp]),
+    "fdadi_hb_capture_solve": (c_int, [c_vp, c_int, c_double, c_i32, c_i32, c_vp]),
     "fdadi_hb_set_shape": (c_int, [c_vp, c_int, c_int, c_int]),
     "fdadi_hb_get_shape": (c_int, [c_vp, c_int, _P(c_int), _P(c_int)]),
     "fdadi_hb_launches_per_step": (c_int, [c_vp, _P(c_int)]),

@@ -228,6 +228,18 @@ class HestonBatch(object):
     def replay(self):
         _lib.call("fdadi_hb_replay", self._h, self.stream)
 
+    def capture_solve(self, scheme, n, theta=0.5, smoothing_steps=0):
+        """Capture ``solve(scheme, n, theta, smoothing_steps)`` as one graph
+        (``replay`` runs it from the current state): with ``smoothing_steps``
+        = s, 2s implicit half steps, then n - s steps of ``scheme``.  Leaves
+        the engine configured for ``scheme``."""
+        n, s = int(n), int(smoothing_steps)
+        if n - s <= 0:
+            raise ValueError("capture_solve: n must exceed the smoothing steps")
+        code = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        _lib.call("fdadi_hb_capture_solve", self._h, code, float(theta), s, n - s, self.stream)
+        self.scheme = code
+
     def sync(self):
         _lib.call("fdadi_stream_sync", self.stream)
 

@@ -1634,10 +1634,11 @@ int build_tables(fdadi_hb *hb, hipStream_t s) {
 // back to the host and nothing synchronises), else from hb->theta /
 // hb->dt_scale (synchronises).
 int refactor(fdadi_hb *hb, hipStream_t s, const double *d_theta = nullptr,
-             const double *d_scale = nullptr) {
+             const double *d_scale = nullptr, bool capturing = false) {
   const bool dev = d_theta || d_scale;
-  // the device copy of the options is current after fdadi_hb_update_dev
-  if (!hb->opts_on_device)
+  // the device copy of the options is current after fdadi_hb_update_dev (and
+  // is made current before a capture, which records kernels only)
+  if (!hb->opts_on_device && !capturing)
     FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
                                  hipMemcpyHostToDevice, s));
   const bool reuse = d_theta == hb->d_slots;  // refactorisation with the kept pair
@@ -1652,8 +1653,19 @@ int refactor(fdadi_hb *hb, hipStream_t s, const double *d_theta = nullptr,
   hipLaunchKernelGGL(spot_factor_kernel, dim3(hb->nftiles), dim3(64), 0, s, ctx_of(hb),
                      hb->d_ftiles, hb->nftiles, hb->d_rd0);
   FDADI_CHECK_LAUNCH();
-  if (!dev) FDADI_TRY_HIP(hipStreamSynchronize(s));
+  if (!dev && !capturing) FDADI_TRY_HIP(hipStreamSynchronize(s));
   return FDADI_OK;
+}
+
+void drop_graph(fdadi_hb *hb) {
+  if (hb->gexec) {
+    (void)hipGraphExecDestroy(hb->gexec);
+    hb->gexec = nullptr;
+  }
+  if (hb->graph) {
+    (void)hipGraphDestroy(hb->graph);
+    hb->graph = nullptr;
+  }
 }
 
 void free_hb(fdadi_hb *hb) {
@@ -2017,6 +2029,57 @@ int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream) {
   }
   if (e != hipSuccess) {
     set_error(std::string("fdadi_hb_capture: ") + hipGetErrorString(e));
+    return FDADI_ERR_HIP;
+  }
+  hb->graph = g;
+  FDADI_TRY_HIP(hipGraphInstantiate(&hb->gexec, g, nullptr, nullptr, 0));
+  return FDADI_OK;
+}
+
+int fdadi_hb_capture_solve(fdadi_hb *hb, int scheme, double theta, int32_t smoothing,
+                           int32_t nsteps, void *stream) {
+  FDADI_REQUIRE(hb, FDADI_ERR_ARG, "fdadi_hb_capture_solve: null engine");
+  FDADI_REQUIRE(scheme >= FDADI_SCHEME_IMPLICIT && scheme <= FDADI_SCHEME_HV, FDADI_ERR_ARG,
+                "fdadi_hb_capture_solve: unknown scheme");
+  FDADI_REQUIRE(smoothing >= 0 && nsteps > 0, FDADI_ERR_ARG,
+                "fdadi_hb_capture_solve: smoothing >= 0 and a positive step count");
+  FDADI_REQUIRE(stream, FDADI_ERR_ARG, "fdadi_hb_capture_solve: needs a created stream");
+  hipStream_t s = as_stream(stream);
+  drop_graph(hb);
+  // the device copy of the options made current here: the capture records
+  // kernels only (no host copy, no synchronisation)
+  if (!hb->opts_on_device) {
+    FDADI_TRY_HIP(hipMemcpyAsync(hb->d_opts, hb->h_opts.data(), sizeof(HOpt) * hb->nopt,
+                                 hipMemcpyHostToDevice, s));
+    FDADI_TRY_HIP(hipStreamSynchronize(s));
+    hb->opts_on_device = true;
+  }
+  hb->dev_scalars = false;
+  FDADI_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int st = FDADI_OK;
+  if (smoothing > 0) {
+    // 2 * smoothing implicit steps of half the step
+    hb->scheme = FDADI_SCHEME_IMPLICIT;
+    hb->theta = 1.0;
+    hb->dt_scale = 0.5;
+    st = refactor(hb, s, nullptr, nullptr, true);
+    for (int32_t n = 0; n < 2 * smoothing && st == FDADI_OK; ++n) st = step_once(hb, s);
+  }
+  if (st == FDADI_OK) {
+    hb->scheme = scheme;
+    hb->theta = theta;
+    hb->dt_scale = 1.0;
+    st = refactor(hb, s, nullptr, nullptr, true);
+  }
+  for (int32_t n = 0; n < nsteps && st == FDADI_OK; ++n) st = step_once(hb, s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (st != FDADI_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("fdadi_hb_capture_solve: ") + hipGetErrorString(e));
     return FDADI_ERR_HIP;
   }
   hb->graph = g;

@@ -242,6 +242,17 @@ int fdadi_hb_step_timed(fdadi_hb *hb, int32_t nsteps, void *stream, double *stag
  * replay it. */
 int fdadi_hb_capture(fdadi_hb *hb, int32_t nsteps, void *stream);
 int fdadi_hb_replay(fdadi_hb *hb, void *stream);
+/* Capture one whole solve into a HIP graph (replaces any previous one; replay
+ * with fdadi_hb_replay from the initial state): with smoothing = s > 0, first
+ * 2s implicit steps of half the step (theta 1), then nsteps steps of
+ * `scheme` with `theta`, each part preceded by its refactorisation -- the
+ * sequence of the reference's smoothed solve
+ * (FiniteDifferenceEngineGPU.pyx:565-593, solve_smooth_: 2s implicit half
+ * steps, then HV with theta 0.6), without a host launch per stage or the
+ * synchronisation of each configure.  Leaves the engine configured for
+ * `scheme`. */
+int fdadi_hb_capture_solve(fdadi_hb *hb, int scheme, double theta, int32_t smoothing,
+                           int32_t nsteps, void *stream);
 /* Launch shape per sweep kind, used by the autotuner (0 = library default):
  * kind 0 (spot sweep): `threads` per work-group (64..512) and `lines` =
  * variance columns per tile (4, 8, 16 or 32; 0 = 4; 8 and 16 need 256 or 512

@@ -83,6 +83,41 @@ def test_graph_replay_equals_eager(fd):
     npt.assert_array_equal(g.get_packed(), ref)
 
 
+def test_whole_solve_graph_with_smoothing(fd):
+    """fdadi_hb_capture_solve: one graph for a smoothed solve (4 implicit half
+    steps, then HV at theta 0.6, each part with its refactorisation), replayed
+    three times from the payoff with eager work between the replays: every
+    replay equals the eager solve bit for bit, and the eager solve matches the
+    oracle's smoothed solve."""
+    P, S, Vs, dts, O = draw(13, 4)
+    init = [o.payoff() for o in O]
+    eager = fd.HestonBatch(P, S, Vs, dts)
+    eager.set_state(init)
+    eager.solve("hv", 20, theta=0.6, smoothing_steps=2)
+    ref = eager.get_packed()
+    got = eager.get_state()
+    for k, o in enumerate(O):
+        want = o.solve_smooth(20, dts[k], o.payoff(), smoothing_steps=2)
+        alt = [a.solve_smooth(20, dts[k], o.payoff(), smoothing_steps=2) for a in alternates(o)]
+        tol, floor = grid_tolerance(want, alt)
+        assert np.max(np.abs(got[k] - want)) <= tol, k
+    g = fd.HestonBatch(P, S, Vs, dts)
+    g.set_state(init)
+    g.solve("hv", 20, theta=0.6, smoothing_steps=2)  # kernel attributes set outside capture
+    g.capture_solve("hv", 20, theta=0.6, smoothing_steps=2)
+    for rep in range(3):
+        g.set_state(init)
+        g.replay()
+        g.sync()
+        npt.assert_array_equal(g.get_packed(), ref)
+        # eager work between the replays of the same graph
+        eager.set_state(init)
+        eager.solve("douglas", 3, theta=0.5)
+        eager.sync()
+    with pytest.raises(ValueError):
+        g.capture_solve("hv", 2, theta=0.6, smoothing_steps=2)
+
+
 def test_launch_shapes_agree(fd):
     P, S, Vs, dts, O = draw(3, 3, nS_range=(40, 100), nv_range=(32, 64))
     outs = []
