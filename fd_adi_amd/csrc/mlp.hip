@@ -37,9 +37,12 @@ constexpr int kInPad = 128;  // padded input width
 constexpr int kH = 512;      // hidden width
 constexpr int kLayers = 4;   // hidden layers (the 512 -> 1 head is separate)
 constexpr int kTile = 128;
-constexpr int kKStep = 32;
+constexpr int kKStep = 64;   // K extent of one LDS stage of the GEMMs
 constexpr int kMaxSplits = 16;  // split-K factor cap for the weight-gradient GEMMs
 constexpr int kLdsPitch = kKStep + 8;  // bf16 elements per LDS row (16 B pad)
+constexpr int kThreads = 512;          // GEMM work-group: 8 waves
+constexpr int kWN = 4;                 // waves along N (2 along M)
+constexpr int kBM = 4, kBN = 2;        // 16 x 16 MFMA blocks per wave: 64 x 32
 
 enum Epi { EPI_FWD = 0, EPI_INF = 1, EPI_BWD = 2, EPI_F32 = 3 };
 
@@ -63,42 +66,45 @@ struct EpiArgs {
 
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 constexpr int kEpPitch = kTile + 8;  // bf16 elements per row of an epilogue tile
-// LDS of the GEMM: the K loop's A and B buffers; the epilogue tiles reuse it
-// (one 128 x 128 tile for EPI_INF, two for EPI_FWD and EPI_BWD).
+// LDS of the GEMM: the K loop's two A and two B stages; the epilogue tiles
+// reuse it (one 128 x 128 tile for EPI_INF, two for EPI_BWD, three for
+// EPI_FWD).
 constexpr int kLoopLds = 4 * kTile * kLdsPitch;
 constexpr int kTileLds = kTile * kEpPitch;
 constexpr int gemm_lds(int epi) {
   return epi == EPI_F32 ? kLoopLds
          : epi == EPI_INF ? (kLoopLds > kTileLds ? kLoopLds : kTileLds)
-                          : (kLoopLds > 2 * kTileLds ? kLoopLds : 2 * kTileLds);
+         : epi == EPI_BWD ? (kLoopLds > 2 * kTileLds ? kLoopLds : 2 * kTileLds)
+                          : (kLoopLds > 3 * kTileLds ? kLoopLds : 3 * kTileLds);
 }
 
-// C = A B^T on MFMA 16x16x32 bf16.  256 threads = 4 waves in a 2 x 2 grid,
-// each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA blocks.  The K loop stages
-// 128 x 32 tiles of A and B in LDS.
+// C = A B^T on MFMA 16x16x32 bf16.  512 threads = 8 waves in a 2 x 4 grid,
+// each wave owns a 64 x 32 sub-tile = 4 x 2 MFMA blocks; two waves per SIMD
+// hide each other's LDS and barrier latency.  The K loop stages 128 x 64
+// tiles of A and B in LDS (two 32-deep MFMA steps per stage).
 // MFMA operand layout (v_mfma_f32_16x16x32_bf16): lane l holds
 //   A[i = l%16][k = 8*(l/16) .. +7],  B[k = 8*(l/16) .. +7][j = l%16],
 //   D[i = 4*(l/16) + t][j = l%16] for t = 0..3.
-// The grids here are about one work-group per CU, one wave per SIMD, so the
-// loads of a K step must be in flight long before it: a ring of kDepth
-// register stages keeps the next kDepth - 1 steps' global loads outstanding
-// while the MFMAs of the current step run (16 MFMAs of a wave take ~256
-// cycles, an L2 load several hundred).  The K extent of a work-group is a
-// multiple of kDepth steps, so the ring needs no remainder code.
-constexpr int kDepth = 4;
+// The grids here are about one work-group per CU, so the loads of a K stage
+// must be in flight long before it: a ring of kDepth register stages keeps
+// the next stage's global loads outstanding while the MFMAs of the current
+// one run.  The K extent of a work-group is a multiple of kDepth stages, so
+// the ring needs no remainder code.
+constexpr int kDepth = 2;
 constexpr int kKBlock = kKStep * kDepth;  // K granularity of the GEMMs
+constexpr int kChunks = kTile * kKStep / 8 / kThreads;  // 16-byte loads per operand per stage
 
 template <int EPI>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kThreads)
 gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, int N, int K,
                EpiArgs ep) {
-  // Two LDS buffers: the MFMAs of K step k read buffer k % 2 while step
-  // k + 1 is written to the other one, so one barrier per step suffices.
+  // Two LDS stages: the MFMAs of stage s read buffer s % 2 while stage s + 1
+  // is written to the other one, so one barrier per stage suffices.
   __shared__ __attribute__((aligned(16))) bf16 smem[gemm_lds(EPI)];
   bf16 *const As[2] = {smem, smem + kTile * kLdsPitch};
   bf16 *const Bs[2] = {smem + 2 * kTile * kLdsPitch, smem + 3 * kTile * kLdsPitch};
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  const int wm = wave / 2, wn = wave % 2;
+  const int wm = wave / kWN, wn = wave % kWN;
   // Work-groups are dealt to the 8 XCDs round-robin by linear id; renumber
   // them so that each XCD gets a contiguous run of tiles (x fastest): the
   // N tiles of one row block of A, and of one split, then share that XCD's L2.
@@ -109,60 +115,65 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
   const int m0 = by * kTile, n0 = bx * kTile;
   // split-K: work-group z covers K columns [z*Ks, (z+1)*Ks) (gridDim.z = 1: all)
   const int Ks = K / (int)gridDim.z, kbeg = bz * Ks, nsteps = Ks / kKStep;
-  f32x4 acc[4][4];
+  f32x4 acc[kBM][kBN];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < kBM; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // Global -> registers -> LDS: each thread moves 2 x 16 B of A and of B per K step.
-  const int lrow = tid / 4, lk = (tid % 4) * 8;
+    for (int b = 0; b < kBN; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // Global -> registers -> LDS: chunk q = tid + kThreads * p of a 128 x 64
+  // tile is row q / 8, elements 8 (q % 8) .. +7.
+  const int lrow = tid / (kKStep / 8), lk = (tid % (kKStep / 8)) * 8;
+  constexpr int kRowStep = kThreads / (kKStep / 8);  // rows between a thread's chunks
   const bf16 *Ag = A + (size_t)(m0 + lrow) * K + kbeg + lk;
   const bf16 *Bg = B + (size_t)(n0 + lrow) * K + kbeg + lk;
-  bf16x8 ra[kDepth][2], rb[kDepth][2];
+  bf16x8 ra[kDepth][kChunks], rb[kDepth][kChunks];
   auto fetch = [&](bf16x8 *xa, bf16x8 *xb, int step) {
     const int k = step * kKStep;
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      xa[p] = *reinterpret_cast<const bf16x8 *>(Ag + (size_t)64 * p * K + k);
-      xb[p] = *reinterpret_cast<const bf16x8 *>(Bg + (size_t)64 * p * K + k);
+    for (int p = 0; p < kChunks; ++p) {
+      xa[p] = *reinterpret_cast<const bf16x8 *>(Ag + (size_t)kRowStep * p * K + k);
+      xb[p] = *reinterpret_cast<const bf16x8 *>(Bg + (size_t)kRowStep * p * K + k);
     }
   };
   auto stage = [&](const bf16x8 *xa, const bf16x8 *xb, int buf) {
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int r = lrow + 64 * p;
+    for (int p = 0; p < kChunks; ++p) {
+      const int r = lrow + kRowStep * p;
       *reinterpret_cast<bf16x8 *>(As[buf] + r * kLdsPitch + lk) = xa[p];
       *reinterpret_cast<bf16x8 *>(Bs[buf] + r * kLdsPitch + lk) = xb[p];
     }
   };
   auto compute = [&](int buf) {
-    bf16x8 af[4], bfr[4];
-    const int kseg = (lane / 16) * 8;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int row = wm * 64 + a * 16 + lane % 16;
-      af[a] = *reinterpret_cast<const bf16x8 *>(As[buf] + row * kLdsPitch + kseg);
+    for (int kk = 0; kk < kKStep; kk += 32) {
+      bf16x8 af[kBM], bfr[kBN];
+      const int kseg = kk + (lane / 16) * 8;
+#pragma unroll
+      for (int a = 0; a < kBM; ++a) {
+        const int row = wm * kBM * 16 + a * 16 + lane % 16;
+        af[a] = *reinterpret_cast<const bf16x8 *>(As[buf] + row * kLdsPitch + kseg);
+      }
+#pragma unroll
+      for (int b = 0; b < kBN; ++b) {
+        const int col = wn * kBN * 16 + b * 16 + lane % 16;
+        bfr[b] = *reinterpret_cast<const bf16x8 *>(Bs[buf] + col * kLdsPitch + kseg);
+      }
+#pragma unroll
+      for (int a = 0; a < kBM; ++a)
+#pragma unroll
+        for (int b = 0; b < kBN; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
     }
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int col = wn * 64 + b * 16 + lane % 16;
-      bfr[b] = *reinterpret_cast<const bf16x8 *>(Bs[buf] + col * kLdsPitch + kseg);
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
   };
 #pragma unroll
   for (int j = 0; j < kDepth; ++j) fetch(ra[j], rb[j], j);
   stage(ra[0], rb[0], 0);
   __syncthreads();
-  // Step s computes from LDS buffer s % 2; its registers (ring slot s %
-  // kDepth) were staged one step earlier and now take step s + kDepth.  The
-  // loop body is free of branches (past the end the loads repeat the last
-  // step and the stage is never read), so the compiler's wait counts keep
-  // kDepth - 1 steps of loads in flight.
+  // Stage s computes from LDS buffer s % 2; its registers (ring slot s %
+  // kDepth) were staged one stage earlier and now take stage s + kDepth.
+  // The loop body is free of branches (past the end the loads repeat the
+  // last stage and the store to LDS is never read), so the compiler's wait
+  // counts keep the next stage's loads in flight.
   for (int s0 = 0; s0 < nsteps; s0 += kDepth) {
 #pragma unroll
     for (int j = 0; j < kDepth; ++j) {
@@ -174,16 +185,17 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
     }
   }
   // Epilogue.  Lane l of wave (wm, wn) holds, for MFMA block (a, b), rows
-  // ml = wm*64 + a*16 + 4*(l/16) + t (t = 0..3) of column nl = wn*64 + b*16 + l%16.
+  // ml = wm*64 + a*16 + 4*(l/16) + t (t = 0..3) of column nl = wn*32 + b*16 + l%16.
+  auto row_of = [&](int a) { return wm * kBM * 16 + a * 16 + 4 * (lane / 16); };
+  auto col_of = [&](int b) { return wn * kBN * 16 + b * 16 + lane % 16; };
   if (EPI == EPI_F32) {
     // split z writes its partial product to slice z of C
     float *Cz = ep.C + (size_t)bz * M * N;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < kBM; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int n = n0 + wn * 64 + b * 16 + lane % 16;
-        const int mb = m0 + wm * 64 + a * 16 + 4 * (lane / 16);
+      for (int b = 0; b < kBN; ++b) {
+        const int n = n0 + col_of(b), mb = m0 + row_of(a);
 #pragma unroll
         for (int t = 0; t < 4; ++t) Cz[(size_t)(mb + t) * N + n] = acc[a][b][t];
       }
@@ -191,100 +203,92 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
   }
   // The bf16 results go through LDS tiles so that every global access is a
   // coalesced 16-byte one: T [ml][nl] (row-major) for out, TT [nl][ml] for
-  // the transposed outT and dT.  The K loop ended with a barrier, so its
-  // buffers are free.
-  bf16 *const T = smem, *const TT = smem + kTileLds;
-  // 2048 chunks of 8 elements per tile, 8 per thread: chunk q is row q / 16,
+  // the transposed outT, DT [nl][ml] for dT.  The K loop ended with a
+  // barrier, so its buffers are free.
+  bf16 *const T = smem, *const TT = smem + kTileLds, *const DT = smem + 2 * kTileLds;
+  // 2048 chunks of 8 elements per tile, 4 per thread: chunk q is row q / 16,
   // elements 8 (q % 16) .. +7 of that row (16 lanes cover one 256-byte row).
+  constexpr int kTileChunks = kTile * kTile / 8 / kThreads;
   auto tile_to_global = [&](const bf16 *tile, bf16 *g, int ld, int r0, int c0) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int q = tid + 256 * i, r = q / 16, c = (q % 16) * 8;
+    for (int i = 0; i < kTileChunks; ++i) {
+      const int q = tid + kThreads * i, r = q / 16, c = (q % 16) * 8;
       *reinterpret_cast<bf16x8 *>(g + (size_t)(r0 + r) * ld + c0 + c) =
           *reinterpret_cast<const bf16x8 *>(tile + r * kEpPitch + c);
     }
   };
-  float r[4][4][4];
+  float r[kBM][kBN][4];
   if (EPI == EPI_BWD) {
     // r = (dZ_next W) * silu'(z): the derivative tile dT [n0.., m0..] into TT
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int q = tid + 256 * i, nl = q / 16, c = (q % 16) * 8;
+    for (int i = 0; i < kTileChunks; ++i) {
+      const int q = tid + kThreads * i, nl = q / 16, c = (q % 16) * 8;
       *reinterpret_cast<bf16x8 *>(TT + nl * kEpPitch + c) =
           *reinterpret_cast<const bf16x8 *>(ep.dT + (size_t)(n0 + nl) * M + m0 + c);
     }
     __syncthreads();
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < kBM; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int nl = wn * 64 + b * 16 + lane % 16, ml = wm * 64 + a * 16 + 4 * (lane / 16);
-        const bf16x4 d = *reinterpret_cast<const bf16x4 *>(TT + nl * kEpPitch + ml);
+      for (int b = 0; b < kBN; ++b) {
+        const bf16x4 d = *reinterpret_cast<const bf16x4 *>(TT + col_of(b) * kEpPitch + row_of(a));
 #pragma unroll
         for (int t = 0; t < 4; ++t) r[a][b][t] = acc[a][b][t] * (float)d[t];
       }
     __syncthreads();
   } else {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const float bn = ep.bias[n0 + wn * 64 + b * 16 + lane % 16];
+    for (int b = 0; b < kBN; ++b) {
+      const float bn = ep.bias[n0 + col_of(b)];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < kBM; ++a)
 #pragma unroll
         for (int t = 0; t < 4; ++t) r[a][b][t] = acc[a][b][t] + bn;  // pre-activation z
     }
   }
-  // activations (or back-propagated gradients) into T, and into TT
+  // activations (or back-propagated gradients) into T and TT; for the
+  // forward pass also silu'(z) into DT
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < kBM; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int nl = wn * 64 + b * 16 + lane % 16, ml = wm * 64 + a * 16 + 4 * (lane / 16);
-      bf16x4 o;
+    for (int b = 0; b < kBN; ++b) {
+      const int nl = col_of(b), ml = row_of(a);
+      bf16x4 o, d;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        o[t] = static_cast<bf16>(EPI == EPI_BWD ? r[a][b][t] : r[a][b][t] * sigmoid(r[a][b][t]));
+        const float z = r[a][b][t];
+        if (EPI == EPI_BWD) {
+          o[t] = static_cast<bf16>(z);
+        } else {
+          const float sg = sigmoid(z);
+          o[t] = static_cast<bf16>(z * sg);
+          if (EPI == EPI_FWD) d[t] = static_cast<bf16>(sg * (1.0f + z * (1.0f - sg)));
+        }
         T[(ml + t) * kEpPitch + nl] = o[t];
       }
       if (EPI != EPI_INF) *reinterpret_cast<bf16x4 *>(TT + nl * kEpPitch + ml) = o;
+      if (EPI == EPI_FWD) *reinterpret_cast<bf16x4 *>(DT + nl * kEpPitch + ml) = d;
     }
   __syncthreads();
   if (EPI == EPI_FWD && ep.ypart) {
-    // the head on this tile's 128 columns: two threads per row, 64 columns each
-    const int row = tid / 2, h = (tid % 2) * 64;
+    // the head on this tile's 128 columns: four threads per row, 32 columns each
+    const int row = tid / 4, h = (tid % 4) * 32;
     float acc_y = 0.f;
 #pragma unroll
-    for (int c = 0; c < 64; c += 8) {
+    for (int c = 0; c < 32; c += 8) {
       const bf16x8 av = *reinterpret_cast<const bf16x8 *>(T + row * kEpPitch + h + c);
       const bf16x8 wv = *reinterpret_cast<const bf16x8 *>(ep.w4 + n0 + h + c);
 #pragma unroll
       for (int t = 0; t < 8; ++t) acc_y += (float)av[t] * (float)wv[t];
     }
     acc_y += __shfl_xor(acc_y, 1, 64);
+    acc_y += __shfl_xor(acc_y, 2, 64);
     if (h == 0) ep.ypart[(size_t)bx * M + m0 + row] = acc_y;
   }
   tile_to_global(T, ep.out, N, m0, n0);
   if (EPI == EPI_INF) return;
   tile_to_global(TT, ep.outT, M, n0, m0);
-  if (EPI == EPI_FWD) {
-    // silu'(z), transposed, for the backward pass
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int nl = wn * 64 + b * 16 + lane % 16, ml = wm * 64 + a * 16 + 4 * (lane / 16);
-        bf16x4 d;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const float z = r[a][b][t], sg = sigmoid(z);
-          d[t] = static_cast<bf16>(sg * (1.0f + z * (1.0f - sg)));
-        }
-        *reinterpret_cast<bf16x4 *>(TT + nl * kEpPitch + ml) = d;
-      }
-    __syncthreads();
-    tile_to_global(TT, ep.dT, M, n0, m0);
-  }
+  if (EPI == EPI_FWD) tile_to_global(DT, ep.dT, M, n0, m0);
 }
 
 // Features -> padded, normalised bf16 input rows:
@@ -580,10 +584,10 @@ int gemm(int epi, const bf16 *A, const bf16 *B, int M, int N, int K, const EpiAr
                 FDADI_ERR_ARG, "mlp gemm: split-K needs the fp32 epilogue and K % (128 s) == 0");
   dim3 grid(N / kTile, M / kTile, splits);
   switch (epi) {
-    case EPI_FWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
-    case EPI_INF: hipLaunchKernelGGL(gemm_nt_kernel<EPI_INF>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
-    case EPI_BWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
-    default: hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, dim3(256), 0, s, A, B, M, N, K, ep); break;
+    case EPI_FWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
+    case EPI_INF: hipLaunchKernelGGL(gemm_nt_kernel<EPI_INF>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
+    case EPI_BWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
+    default: hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
   }
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;

@@ -48,10 +48,11 @@ __global__ void forest_predict_kernel(const FNode *__restrict__ nodes,
 // ---- ensemble combination and gate features ----------------------------------------
 // price = baseline + mean of the M members' corrections (summed member by
 // member); spread = their standard deviation (M - 1 in the denominator);
-// the error model's features: log of kappa, theta, sigma, v0, T, K/S0; rho
-// and r as they are; log Feller ratio 2 kappa theta / sigma^2; log10 of
-// spread + 1e-12.  inside = the row lies in the box [lo, hi] and its Feller
-// ratio is at least feller_min.
+// the error model's 13 features: log of kappa, theta, sigma, v0, T, K/S0;
+// rho and r as they are; log Feller ratio 2 kappa theta / sigma^2; log10 of
+// spread + 1e-12; the mean correction; log10 of its magnitude + 1e-9; log10
+// of the members' range (max - min) + 1e-12.  inside = the row lies in the
+// box [lo, hi] and its Feller ratio is at least feller_min.
 struct Box {
   double lo[8], hi[8];
 };
