@@ -59,8 +59,10 @@ def dsilu(z):
 
 
 def features(P, lo, hi):
+    """Normalised input rows, zero-padded to 128 columns (P has up to 16
+    feature columns; 8 for the Heston surrogate)."""
     X = np.zeros((P.shape[0], N_PAD))
-    X[:, :N_IN] = 2.0 * (P - lo) / (hi - lo) - 1.0
+    X[:, :P.shape[1]] = 2.0 * (P - lo) / (hi - lo) - 1.0
     return X
 
 

@@ -36,18 +36,20 @@ def test_gemm_nt_matches_numpy(fd):
     npt.assert_allclose(got, want, rtol=1e-5, atol=1e-4)
 
 
-@pytest.mark.parametrize("B", [256, 2048, 8192])
-def test_gradients_match_reference(fd, B):
+@pytest.mark.parametrize("B,nin", [(256, 8), (2048, 8), (8192, 8), (1000, 13)])
+def test_gradients_match_reference(fd, B, nin):
     """B = 2048 exercises the split-K weight-gradient GEMMs (4 splits); 8192
-    is the training batch of the surrogate, 16 splits of 512 rows."""
+    is the training batch of the surrogate, 16 splits of 512 rows; 1000 rows
+    of 13 features (the error model's input) pad the batch to 1024 rows and
+    fill a second 8-column chunk of the input."""
     from fd_adi_amd.surrogate import MLP, init_params
     from fd_adi_amd.SizedArrayPtr import SizedArrayPtr
     rng = np.random.default_rng(1)
-    lo, hi = np.zeros(8), np.ones(8)
-    p = init_params(3)
-    P = rng.uniform(0, 1, (B, 8))
+    lo, hi = -np.ones(nin), 2 * np.ones(nin)
+    p = init_params(3, nin)
+    P = rng.uniform(-1, 2, (B, nin))
     t = np.sin(P.sum(axis=1))
-    m = MLP(lo, hi, 0.0, 1.0, params=p, max_rows=B)
+    m = MLP(lo, hi, 0.0, 1.0, params=p, max_rows=(B + 127) // 128 * 128)
     loss = m.grad(SizedArrayPtr(P), SizedArrayPtr(t), None, B)
     g = m.grads().astype(np.float64)
     X = R.features(P, lo, hi)
