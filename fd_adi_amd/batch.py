@@ -129,15 +129,16 @@ class HestonBatch(object):
     def set_shape(self, kind, threads, lines=0):
         """Launch shape of a sweep kind (0 = spot: threads per work-group and
         ``lines`` = variance columns per tile, 4, 8, 16 or 32; 1 = variance: lanes
-        per line); 0 selects the default."""
+        per line); 0 selects the default, -1 a shape per option from its line
+        length (options of one shape share a launch)."""
         _lib.call("fdadi_hb_set_shape", self._h, int(kind), int(threads), int(lines))
 
     # launch-shape candidates per sweep kind (spot: (threads per work-group,
     # columns per tile); variance: (lanes per line, 0)); set_shape rejects
     # those too small for the grids
     SHAPES = {0: ((64, 4), (128, 4), (256, 4), (512, 4), (256, 8), (512, 8), (256, 16),
-                  (512, 16), (512, 32)),
-              1: tuple((t, 0) for t in (1, 2, 4, 8, 16, 32, 64))}
+                  (512, 16), (512, 32), (-1, 4), (-1, 8), (-1, 16)),
+              1: tuple((t, 0) for t in (1, 2, 4, 8, 16, 32, 64, -1))}
 
     def autotune(self, steps=2):
         """Pick the fastest launch shape of each sweep kind for the configured

@@ -1344,6 +1344,16 @@ struct fdadi_hb {
   std::vector<int> order;           // options, largest first
   int spot_maxl = 8;  // rows per thread in the spot sweep (8, or 16 for long lines)
   int spot_nl = 4;    // columns per spot-sweep tile (4, 8, 16 or 32)
+  // Launch groups of each sweep: a shape and a range of its tile list.  One
+  // group, unless the shape is chosen per option from its line length
+  // (fdadi_hb_set_shape with threads = -1): then options of one shape share
+  // a group, and a stage launches once per group.
+  struct SweepGroup {
+    int threads, maxl;  // spot: threads and rows per thread; variance: lanes per line, -
+    int first, ntiles;
+  };
+  std::vector<SweepGroup> sgroups, vgroups;
+  bool spot_split = false, var_split = false;
   int scheme = -1;
   double theta = 0.5, dt_scale = 1.0;
   bool dev_scalars = false;        // last configure took theta and dt from device memory
@@ -1417,10 +1427,37 @@ int spot_shape(int64_t n, int threads, int nl, int *t_out, int *maxl_out) {
   return FDADI_OK;
 }
 
-// (Re)build the spot-sweep tile list: spot_nl columns per tile.
+// Spot shapes (threads, rows per thread) for nl-column tiles, by the spot
+// lines they cover ((threads / nl) * rows), smallest first.
+std::vector<std::pair<int, int>> spot_candidates(int nl) {
+  std::vector<std::pair<int, int>> c;
+  for (int t : {64, 128, 256, 512})
+    for (int ml : {kMAXLs, 2 * kMAXLs}) {
+      // the launch shapes spot_shape accepts
+      if (t / nl < 1 || (nl != 4 && t < 256) || (nl == 32 && t != 512)) continue;
+      if (ml == 2 * kMAXLs && t != 256 && t != 512) continue;
+      c.push_back({t, ml});
+    }
+  std::stable_sort(c.begin(), c.end(), [nl](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+    return a.first / nl * a.second < b.first / nl * b.second;
+  });
+  return c;
+}
+
+// The shape a split spot sweep gives option o: the smallest candidate whose
+// lines cover its nS (-1 if none does).
+int spot_group_of(const std::vector<std::pair<int, int>> &c, int nl, int64_t nS) {
+  for (size_t g = 0; g < c.size(); ++g)
+    if ((int64_t)(c[g].first / nl) * c[g].second >= nS) return (int)g;
+  return -1;
+}
+
+// (Re)build the spot-sweep tile list (spot_nl columns per tile) and its
+// launch groups.
 int build_spot_tiles(fdadi_hb *hb) {
   std::vector<Tile> st;
-  for (int k : hb->order) {
+  hb->sgroups.clear();
+  auto add_tiles = [&](int k) {
     const HOpt &o = hb->h_opts[k];
     // the last tile ends on the last column (it may overlap the one before,
     // whose shared columns both compute identically): the kernel takes
@@ -1428,6 +1465,19 @@ int build_spot_tiles(fdadi_hb *hb) {
     for (int64_t j0 = 0; j0 < o.nv; j0 += hb->spot_nl) {
       const int64_t start = j0 + hb->spot_nl > o.nv && o.nv >= hb->spot_nl ? o.nv - hb->spot_nl : j0;
       st.push_back(Tile{k, (int32_t)start});
+    }
+  };
+  if (!hb->spot_split) {
+    for (int k : hb->order) add_tiles(k);
+    hb->sgroups.push_back({hb->spot_threads, hb->spot_maxl, 0, (int)st.size()});
+  } else {
+    const auto c = spot_candidates(hb->spot_nl);
+    for (size_t g = 0; g < c.size(); ++g) {
+      const int first = (int)st.size();
+      for (int k : hb->order)  // largest first within a group
+        if (spot_group_of(c, hb->spot_nl, hb->h_opts[k].nS) == (int)g) add_tiles(k);
+      if ((int)st.size() > first)
+        hb->sgroups.push_back({c[g].first, c[g].second, first, (int)st.size() - first});
     }
   }
   FDADI_REQUIRE((int64_t)st.size() <= hb->cap_stiles, FDADI_ERR_STATE,
@@ -1454,12 +1504,26 @@ int var_tpl_for(int64_t nv) {
   return t;
 }
 
-// (Re)build the variance-sweep tile list: var_rows spot rows per tile.
+// (Re)build the variance-sweep tile list (256 / lanes-per-line spot rows per
+// tile) and its launch groups; split: each option gets the fewest lanes
+// whose chunks of kMAXL positions cover its variance line.
 int build_var_tiles(fdadi_hb *hb) {
   std::vector<Tile> vtl;
-  for (int k : hb->order) {
+  hb->vgroups.clear();
+  auto add_tiles = [&](int k, int rows) {
     const HOpt &o = hb->h_opts[k];
-    for (int64_t i0 = 0; i0 < o.nS; i0 += hb->var_rows) vtl.push_back(Tile{k, (int32_t)i0});
+    for (int64_t i0 = 0; i0 < o.nS; i0 += rows) vtl.push_back(Tile{k, (int32_t)i0});
+  };
+  if (!hb->var_split) {
+    for (int k : hb->order) add_tiles(k, hb->var_rows);
+    hb->vgroups.push_back({hb->var_tpl, 0, 0, (int)vtl.size()});
+  } else {
+    for (int t = 1; t <= 64; t *= 2) {
+      const int first = (int)vtl.size();
+      for (int k : hb->order)
+        if (var_tpl_for(hb->h_opts[k].nv) == t) add_tiles(k, 256 / t);
+      if ((int)vtl.size() > first) hb->vgroups.push_back({t, 0, first, (int)vtl.size() - first});
+    }
   }
   FDADI_REQUIRE((int64_t)vtl.size() <= hb->cap_vtiles, FDADI_ERR_STATE,
                 "variance tile list exceeds its capacity");
@@ -1478,8 +1542,8 @@ int threads_for(int64_t n, int nl = kNL, int maxl = kMAXL) {
 }
 
 template <int MODE, int T, int NL, int ML>
-int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *out,
-                  hipStream_t s) {
+int launch_spot_t(const fdadi_hb *hb, const fdadi_hb::SweepGroup &g, const double *U,
+                  const double *Y, double *out, hipStream_t s) {
   constexpr int TPL = T / NL;
   // chunk carries of the scans, the next stage's explicit term (MAXL per
   // thread); wide tiles add their table rows (MAXL rows of 12 fields + 2
@@ -1488,78 +1552,93 @@ int launch_spot_t(const fdadi_hb *hb, const double *U, const double *Y, double *
   const size_t tab = NL == 4 ? 0 : (size_t)TPL * (ML * 12 + 2);
   const size_t lds = sizeof(double) * (3 * (size_t)NL * (TPL + 1) + ex + tab);
   FDADI_RET(allow_lds((const void *)spot_sweep_kernel<MODE, T, NL, ML>, lds));
-  hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, NL, ML>), dim3(hb->nstiles), dim3(T), lds, s,
-                     ctx_of(hb), hb->d_stiles, U, Y, out);
+  hipLaunchKernelGGL((spot_sweep_kernel<MODE, T, NL, ML>), dim3(g.ntiles), dim3(T), lds, s,
+                     ctx_of(hb), hb->d_stiles + g.first, U, Y, out);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
 
 template <int MODE, int NL>
-int launch_spot_wide(const fdadi_hb *hb, const double *U, const double *Y, double *out,
-                     hipStream_t s) {
-  const bool long_rows = hb->spot_maxl == 2 * kMAXLs;
+int launch_spot_wide(const fdadi_hb *hb, const fdadi_hb::SweepGroup &g, const double *U,
+                     const double *Y, double *out, hipStream_t s) {
+  const bool long_rows = g.maxl == 2 * kMAXLs;
   if constexpr (NL != 32) {  // 32-column tiles: 512 threads (16 lanes per line)
-    switch (hb->spot_threads * 2 + (long_rows ? 1 : 0)) {
-      case 512: return launch_spot_t<MODE, 256, NL, kMAXLs>(hb, U, Y, out, s);
-      case 513: return launch_spot_t<MODE, 256, NL, 2 * kMAXLs>(hb, U, Y, out, s);
+    switch (g.threads * 2 + (long_rows ? 1 : 0)) {
+      case 512: return launch_spot_t<MODE, 256, NL, kMAXLs>(hb, g, U, Y, out, s);
+      case 513: return launch_spot_t<MODE, 256, NL, 2 * kMAXLs>(hb, g, U, Y, out, s);
     }
   }
-  switch (hb->spot_threads * 2 + (long_rows ? 1 : 0)) {
-    case 1024: return launch_spot_t<MODE, 512, NL, kMAXLs>(hb, U, Y, out, s);
-    case 1025: return launch_spot_t<MODE, 512, NL, 2 * kMAXLs>(hb, U, Y, out, s);
+  switch (g.threads * 2 + (long_rows ? 1 : 0)) {
+    case 1024: return launch_spot_t<MODE, 512, NL, kMAXLs>(hb, g, U, Y, out, s);
+    case 1025: return launch_spot_t<MODE, 512, NL, 2 * kMAXLs>(hb, g, U, Y, out, s);
   }
   set_error("spot sweep: unsupported shape for 8-, 16- or 32-column tiles");
   return FDADI_ERR_UNSUPPORTED;
 }
 
 template <int MODE>
-int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *out,
-                hipStream_t s) {
-  if (hb->spot_nl == 32) return launch_spot_wide<MODE, 32>(hb, U, Y, out, s);
-  if (hb->spot_nl == 16) return launch_spot_wide<MODE, 16>(hb, U, Y, out, s);
-  if (hb->spot_nl == 8) return launch_spot_wide<MODE, 8>(hb, U, Y, out, s);
-  if (hb->spot_maxl == 2 * kMAXLs) {
-    switch (hb->spot_threads) {
-      case 256: return launch_spot_t<MODE, 256, 4, 2 * kMAXLs>(hb, U, Y, out, s);
-      case 512: return launch_spot_t<MODE, 512, 4, 2 * kMAXLs>(hb, U, Y, out, s);
+int launch_spot_group(const fdadi_hb *hb, const fdadi_hb::SweepGroup &g, const double *U,
+                      const double *Y, double *out, hipStream_t s) {
+  if (hb->spot_nl == 32) return launch_spot_wide<MODE, 32>(hb, g, U, Y, out, s);
+  if (hb->spot_nl == 16) return launch_spot_wide<MODE, 16>(hb, g, U, Y, out, s);
+  if (hb->spot_nl == 8) return launch_spot_wide<MODE, 8>(hb, g, U, Y, out, s);
+  if (g.maxl == 2 * kMAXLs) {
+    switch (g.threads) {
+      case 256: return launch_spot_t<MODE, 256, 4, 2 * kMAXLs>(hb, g, U, Y, out, s);
+      case 512: return launch_spot_t<MODE, 512, 4, 2 * kMAXLs>(hb, g, U, Y, out, s);
     }
     set_error("spot sweep: unsupported thread count for 16 rows per thread");
     return FDADI_ERR_UNSUPPORTED;
   }
-  switch (hb->spot_threads) {
-    case 64: return launch_spot_t<MODE, 64, 4, kMAXLs>(hb, U, Y, out, s);
-    case 128: return launch_spot_t<MODE, 128, 4, kMAXLs>(hb, U, Y, out, s);
-    case 256: return launch_spot_t<MODE, 256, 4, kMAXLs>(hb, U, Y, out, s);
-    case 512: return launch_spot_t<MODE, 512, 4, kMAXLs>(hb, U, Y, out, s);
+  switch (g.threads) {
+    case 64: return launch_spot_t<MODE, 64, 4, kMAXLs>(hb, g, U, Y, out, s);
+    case 128: return launch_spot_t<MODE, 128, 4, kMAXLs>(hb, g, U, Y, out, s);
+    case 256: return launch_spot_t<MODE, 256, 4, kMAXLs>(hb, g, U, Y, out, s);
+    case 512: return launch_spot_t<MODE, 512, 4, kMAXLs>(hb, g, U, Y, out, s);
   }
   set_error("spot sweep: unsupported thread count");
   return FDADI_ERR_UNSUPPORTED;
 }
 
+// One spot-sweep stage: a launch per group.
+template <int MODE>
+int launch_spot(const fdadi_hb *hb, const double *U, const double *Y, double *out,
+                hipStream_t s) {
+  for (const fdadi_hb::SweepGroup &g : hb->sgroups)
+    FDADI_RET(launch_spot_group<MODE>(hb, g, U, Y, out, s));
+  return FDADI_OK;
+}
+
 template <int MODE, int TPL>
-int launch_var_t(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
+int launch_var_t(const fdadi_hb *hb, const fdadi_hb::SweepGroup &g, const double *W, double *out,
+                 hipStream_t s) {
   // LDS: three tables and 256/TPL lines of TPL chunks (pitch kMAXL + 1) + 1.
   const size_t lds = sizeof(double) * (3 + 256 / TPL) * (TPL * (kMAXL + 1) + 1);
   FDADI_RET(allow_lds((const void *)var_sweep_kernel<MODE, TPL, kMAXL>, lds));
-  hipLaunchKernelGGL((var_sweep_kernel<MODE, TPL, kMAXL>), dim3(hb->nvtiles), dim3(256), lds, s,
-                     ctx_of(hb), hb->d_vtiles, W, out);
+  hipLaunchKernelGGL((var_sweep_kernel<MODE, TPL, kMAXL>), dim3(g.ntiles), dim3(256), lds, s,
+                     ctx_of(hb), hb->d_vtiles + g.first, W, out);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
 
+// One variance-sweep stage: a launch per group.
 template <int MODE>
 int launch_var(const fdadi_hb *hb, const double *W, double *out, hipStream_t s) {
-  switch (hb->var_tpl) {
-    case 1: return launch_var_t<MODE, 1>(hb, W, out, s);
-    case 2: return launch_var_t<MODE, 2>(hb, W, out, s);
-    case 4: return launch_var_t<MODE, 4>(hb, W, out, s);
-    case 8: return launch_var_t<MODE, 8>(hb, W, out, s);
-    case 16: return launch_var_t<MODE, 16>(hb, W, out, s);
-    case 32: return launch_var_t<MODE, 32>(hb, W, out, s);
-    case 64: return launch_var_t<MODE, 64>(hb, W, out, s);
+  for (const fdadi_hb::SweepGroup &g : hb->vgroups) {
+    int st = FDADI_ERR_UNSUPPORTED;
+    switch (g.threads) {
+      case 1: st = launch_var_t<MODE, 1>(hb, g, W, out, s); break;
+      case 2: st = launch_var_t<MODE, 2>(hb, g, W, out, s); break;
+      case 4: st = launch_var_t<MODE, 4>(hb, g, W, out, s); break;
+      case 8: st = launch_var_t<MODE, 8>(hb, g, W, out, s); break;
+      case 16: st = launch_var_t<MODE, 16>(hb, g, W, out, s); break;
+      case 32: st = launch_var_t<MODE, 32>(hb, g, W, out, s); break;
+      case 64: st = launch_var_t<MODE, 64>(hb, g, W, out, s); break;
+      default: set_error("variance sweep: unsupported lanes per line");
+    }
+    FDADI_RET(st);
   }
-  set_error("variance sweep: unsupported lanes per line");
-  return FDADI_ERR_UNSUPPORTED;
+  return FDADI_OK;
 }
 
 // Optional per-stage event pairs (fdadi_hb_step_timed); null = untimed.
@@ -1979,8 +2058,10 @@ int fdadi_hb_step_timed(fdadi_hb *hb, int32_t nsteps, void *stream, double *stag
                         int32_t max_stages, int32_t *nstages) {
   FDADI_REQUIRE(hb && stage_ms && nstages && nsteps > 0, FDADI_ERR_ARG,
                 "fdadi_hb_step_timed: bad arguments");
-  int per = 0;
-  FDADI_RET(fdadi_hb_launches_per_step(hb, &per));
+  // ADI stages per step (a stage is one launch per group of its sweep)
+  FDADI_REQUIRE(hb->scheme >= 0, FDADI_ERR_STATE, "fdadi_hb_step_timed: engine not configured");
+  const int per =
+      hb->scheme == FDADI_SCHEME_DOUGLAS || hb->scheme == FDADI_SCHEME_IMPLICIT ? 2 : 4;
   FDADI_REQUIRE(per <= max_stages, FDADI_ERR_ARG, "fdadi_hb_step_timed: stage_ms too short");
   hipStream_t s = as_stream(stream);
   const size_t need = (size_t)2 * per * nsteps;
@@ -2098,34 +2179,50 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
   FDADI_REQUIRE(kind == 0 || lines == 0, FDADI_ERR_UNSUPPORTED,
                 "fdadi_hb_set_shape: variance tiles follow from the lanes per line (pass 0)");
   if (kind == 0) {
-    FDADI_REQUIRE(threads == 0 || threads == 64 || threads == 128 || threads == 256 ||
-                      threads == 512,
+    FDADI_REQUIRE(threads == -1 || threads == 0 || threads == 64 || threads == 128 ||
+                      threads == 256 || threads == 512,
                   FDADI_ERR_ARG,
-                  "fdadi_hb_set_shape: threads must be a power of two in [64, 512]");
+                  "fdadi_hb_set_shape: threads must be a power of two in [64, 512], 0 or -1");
     const int nl = lines == 0 ? 4 : lines;
-    int t = 0, ml = 0;
-    FDADI_RET(spot_shape(hb->nSmax, threads, nl, &t, &ml));
     if (nl == 32) {
       // the last column must fall on a tile's last lane (see from_prev2_col)
       for (const HOpt &o : hb->h_opts)
         FDADI_REQUIRE(o.nv >= 32, FDADI_ERR_ARG,
                       "fdadi_hb_set_shape: 32-column spot tiles need at least 32 variance nodes");
     }
-    hb->spot_threads = t;
-    hb->spot_maxl = ml;
-    if (nl != hb->spot_nl) {
-      hb->spot_nl = nl;
-      FDADI_RET(build_spot_tiles(hb));
+    if (threads == -1) {
+      int t = 0, ml = 0;
+      FDADI_RET(spot_shape(1, 0, nl, &t, &ml));  // validates nl
+      const auto c = spot_candidates(nl);
+      for (const HOpt &o : hb->h_opts)
+        FDADI_REQUIRE(spot_group_of(c, nl, o.nS) >= 0, FDADI_ERR_ARG,
+                      "fdadi_hb_set_shape: no spot shape covers a line of " +
+                          std::to_string(o.nS) + " nodes");
+      hb->spot_split = true;
+    } else {
+      int t = 0, ml = 0;
+      FDADI_RET(spot_shape(hb->nSmax, threads, nl, &t, &ml));
+      hb->spot_split = false;
+      hb->spot_threads = t;
+      hb->spot_maxl = ml;
     }
+    hb->spot_nl = nl;
+    FDADI_RET(build_spot_tiles(hb));
   } else {
-    // kind 1: `threads` = lanes per variance line (power of two <= 64)
-    const int t = threads == 0 ? var_tpl_for(hb->nvmax) : threads;
-    FDADI_REQUIRE(t >= 1 && t <= 64 && (t & (t - 1)) == 0, FDADI_ERR_ARG,
-                  "fdadi_hb_set_shape: lanes per variance line must be a power of two <= 64");
-    FDADI_REQUIRE((int64_t)t * kMAXL >= hb->nvmax, FDADI_ERR_ARG,
-                  "fdadi_hb_set_shape: too few lanes for the longest variance line");
-    hb->var_tpl = t;
-    hb->var_rows = 256 / t;
+    // kind 1: `threads` = lanes per variance line (power of two <= 64), or
+    // -1: per option, the fewest lanes that cover its line
+    if (threads == -1) {
+      hb->var_split = true;
+    } else {
+      const int t = threads == 0 ? var_tpl_for(hb->nvmax) : threads;
+      FDADI_REQUIRE(t >= 1 && t <= 64 && (t & (t - 1)) == 0, FDADI_ERR_ARG,
+                    "fdadi_hb_set_shape: lanes per variance line must be a power of two <= 64");
+      FDADI_REQUIRE((int64_t)t * kMAXL >= hb->nvmax, FDADI_ERR_ARG,
+                    "fdadi_hb_set_shape: too few lanes for the longest variance line");
+      hb->var_split = false;
+      hb->var_tpl = t;
+      hb->var_rows = 256 / t;
+    }
     FDADI_RET(build_var_tiles(hb));
   }
   if (hb->gexec) {
@@ -2141,17 +2238,21 @@ int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines) {
 
 int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines) {
   FDADI_REQUIRE(hb && (kind == 0 || kind == 1), FDADI_ERR_ARG, "fdadi_hb_get_shape: bad kind");
-  if (threads) *threads = kind == 0 ? hb->spot_threads : hb->var_tpl;
-  if (lines) *lines = kind == 0 ? hb->spot_nl : hb->var_rows;
+  if (threads)
+    *threads = kind == 0 ? (hb->spot_split ? -1 : hb->spot_threads)
+                         : (hb->var_split ? -1 : hb->var_tpl);
+  if (lines) *lines = kind == 0 ? hb->spot_nl : (hb->var_split ? 0 : hb->var_rows);
   return FDADI_OK;
 }
 
 int fdadi_hb_launches_per_step(const fdadi_hb *hb, int *n) {
   FDADI_REQUIRE(hb && n, FDADI_ERR_ARG, "fdadi_hb_launches_per_step: bad arguments");
+  // one launch per group of each sweep stage
+  const int pair = (int)(hb->sgroups.size() + hb->vgroups.size());
   switch (hb->scheme) {
     case FDADI_SCHEME_DOUGLAS:
-    case FDADI_SCHEME_IMPLICIT: *n = 2; break;
-    default: *n = 4; break;
+    case FDADI_SCHEME_IMPLICIT: *n = pair; break;
+    default: *n = 2 * pair; break;
   }
   return FDADI_OK;
 }

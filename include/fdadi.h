@@ -233,9 +233,10 @@ int fdadi_hb_state(fdadi_hb *hb, double **d_state, int64_t *total);
 int fdadi_hb_offsets(const fdadi_hb *hb, int64_t *offsets);
 /* Advance every option by nsteps steps of the configured scheme. */
 int fdadi_hb_step(fdadi_hb *hb, int32_t nsteps, void *stream);
-/* Advance nsteps steps with a HIP event pair around every stage launch and
- * return, per stage slot of a step (launch order), the summed device time in
- * milliseconds; *nstages = launches per step.  Synchronises. */
+/* Advance nsteps steps with a HIP event pair around every ADI stage (the
+ * launches of its sweep's groups) and return, per stage slot of a step
+ * (launch order), the summed device time in milliseconds; *nstages = stages
+ * per step (2 for Douglas and implicit, 4 otherwise).  Synchronises. */
 int fdadi_hb_step_timed(fdadi_hb *hb, int32_t nsteps, void *stream, double *stage_ms,
                         int32_t max_stages, int32_t *nstages);
 /* Capture nsteps steps into a HIP graph (replaces any previous one) and
@@ -259,11 +260,17 @@ int fdadi_hb_capture_solve(fdadi_hb *hb, int scheme, double theta, int32_t smoot
  * threads, 32 needs 512 threads and at least 32 variance nodes in every
  * option);
  * kind 1 (variance sweep): `threads` = lanes per variance line (1..64, the
- * work-group has 256 threads), `lines` must be 0.  get_shape reports the
- * columns (kind 0) / spot rows (kind 1) per tile.  Discards a captured graph. */
+ * work-group has 256 threads), `lines` must be 0.
+ * threads = -1 (either kind): a shape per option, the smallest whose chunks
+ * cover its line (a batch of uneven grids then does no work for positions a
+ * larger option's shape would add); options of one shape share a launch, so
+ * a stage is one launch per shape in use.  get_shape reports threads (-1
+ * when per option) and the columns (kind 0) / spot rows (kind 1, 0 when per
+ * option) per tile.  Discards a captured graph. */
 int fdadi_hb_set_shape(fdadi_hb *hb, int kind, int threads, int lines);
 int fdadi_hb_get_shape(const fdadi_hb *hb, int kind, int *threads, int *lines);
-/* Per-stage launch list for profiling: number of kernel launches per step. */
+/* Kernel launches per step: one per launch group of each stage's sweep (one
+ * group unless shapes are chosen per option, fdadi_hb_set_shape). */
 int fdadi_hb_launches_per_step(const fdadi_hb *hb, int *n);
 
 /* ---- learned price surrogate ----------------------------------------------------

@@ -122,7 +122,8 @@ def test_launch_shapes_agree(fd):
     P, S, Vs, dts, O = draw(3, 3, nS_range=(40, 100), nv_range=(32, 64))
     outs = []
     for ts, nl, tv in ((0, 0, 0), (256, 4, 16), (512, 4, 64), (256, 8, 16), (512, 8, 8),
-                       (256, 16, 8), (512, 16, 32), (512, 32, 16)):
+                       (256, 16, 8), (512, 16, 32), (512, 32, 16), (-1, 4, -1), (-1, 16, 16),
+                       (-1, 8, -1)):
         hb = fd.HestonBatch(P, S, Vs, dts)
         if ts:
             hb.set_shape(0, ts, nl)
@@ -136,6 +137,55 @@ def test_launch_shapes_agree(fd):
     for x in outs[1:]:
         # chunking changes the summation order of the line solves only
         assert np.max(np.abs(x - outs[0])) <= 1e-12 * scale
+
+
+def test_per_option_shapes_match_oracle(fd):
+    """Launch shapes chosen per option (threads = -1): options whose spot
+    lines (40 .. 500 nodes) and variance lines (20 .. 170) need different
+    shapes go to different launches of one stage.  Craig-Sneyd, 4- and
+    16-column tiles: every option agrees with the one-shape launch to the
+    chunking's rounding (1e-12 of the grid's scale), and with the oracle.
+    (The grids are ones the scheme is stable on: with rho = -0.6, or a
+    variance mesh much finer than the spot mesh, the reference scheme
+    amplifies rounding differences as on the 2048 x 1024 grid.)"""
+    P, S, Vs, dts, O = [], [], [], [], []
+    rng = np.random.default_rng(14)
+    for nS, nv in ((40, 60), (70, 20), (130, 90), (260, 40), (500, 120), (300, 33), (450, 170)):
+        K, theta = 100 * rng.uniform(0.8, 1.2), rng.uniform(0.02, 0.1)
+        p = dict(r=0.03, kappa=2.0, theta=theta, sigma=0.4, rho=-0.3)
+        spots = sinh_space(K, 5 * K, 7.0, nS)
+        vars_ = exponential_space(0, theta, 5 * theta, 4.0, nv)
+        P.append(p)
+        S.append(spots)
+        Vs.append(vars_)
+        dts.append(1.0 / 10)
+        O.append(HestonOracle(HestonParams(spot=100, strike=K, r=p["r"], v0=theta, T=1.0,
+                                           kappa=p["kappa"], theta=theta, sigma=p["sigma"],
+                                           rho=p["rho"]), spots, vars_))
+    want = [o.solve_craigsneyd(10, dts[k], o.payoff(), theta=0.5) for k, o in enumerate(O)]
+    alts = [[a.solve_craigsneyd(10, dts[k], o.payoff(), theta=0.5) for a in alternates(o)]
+            for k, o in enumerate(O)]
+    one = fd.HestonBatch(P, S, Vs, dts)
+    one.set_state([o.payoff() for o in O])
+    one.configure("cs", 0.5)
+    one.step(10)
+    ref = one.get_state()
+    for nl in (4, 16):
+        hb = fd.HestonBatch(P, S, Vs, dts)
+        hb.set_shape(0, -1, nl)
+        hb.set_shape(1, -1)
+        assert hb.get_shape(0) == (-1, nl) and hb.get_shape(1) == (-1, 0)
+        hb.set_state([o.payoff() for o in O])
+        hb.configure("cs", 0.5)
+        assert hb.launches_per_step() > 4  # more than one launch per stage
+        hb.step(10)
+        got = hb.get_state()
+        for k in range(len(O)):
+            scale = np.max(np.abs(ref[k]))
+            assert np.max(np.abs(got[k] - ref[k])) <= 1e-12 * scale, (nl, k)
+            tol, floor = grid_tolerance(want[k], alts[k])
+            err = float(np.max(np.abs(got[k] - want[k])))
+            assert err <= tol, (nl, k, got[k].shape, err, tol)
 
 
 def test_every_column_residue_every_shape_matches_oracle(fd):
@@ -162,10 +212,12 @@ def test_every_column_residue_every_shape_matches_oracle(fd):
     alts = [[a.solve_craigsneyd(10, dts[k], o.payoff(), theta=0.5) for a in alternates(o)]
             for k, o in enumerate(O)]
     for shape in ((0, 0), (64, 4), (128, 4), (256, 4), (256, 8), (512, 8), (256, 16),
-                  (512, 16)):
+                  (512, 16), (-1, 4), (-1, 16)):
         hb = fd.HestonBatch(P, S, Vs, dts)
         if shape[0]:
             hb.set_shape(0, *shape)
+        if shape[0] < 0:
+            hb.set_shape(1, -1)
         hb.set_state([o.payoff() for o in O])
         hb.configure("cs", 0.5)
         hb.step(10)
