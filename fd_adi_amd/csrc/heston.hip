@@ -1372,6 +1372,7 @@ namespace {
 constexpr int kNL = 8;     // variance sweep: rows per tile
 constexpr int kMAXL = 16;  // variance sweep: positions per thread
 constexpr int kMAXLs = 8;  // spot sweep: rows per thread
+constexpr int kMAXLs3 = 12;  // spot sweep, 16-column tiles chosen per option: rows per thread
 
 Ctx ctx_of(const fdadi_hb *hb) {
   Ctx c;
@@ -1432,10 +1433,12 @@ int spot_shape(int64_t n, int threads, int nl, int *t_out, int *maxl_out) {
 std::vector<std::pair<int, int>> spot_candidates(int nl) {
   std::vector<std::pair<int, int>> c;
   for (int t : {64, 128, 256, 512})
-    for (int ml : {kMAXLs, 2 * kMAXLs}) {
-      // the launch shapes spot_shape accepts
+    for (int ml : {kMAXLs, kMAXLs3, 2 * kMAXLs}) {
+      // the launch shapes spot_shape accepts, and 12 rows per thread for
+      // 16-column tiles (the per-option groups' in-between size)
       if (t / nl < 1 || (nl != 4 && t < 256) || (nl == 32 && t != 512)) continue;
-      if (ml == 2 * kMAXLs && t != 256 && t != 512) continue;
+      if (ml != kMAXLs && t != 256 && t != 512) continue;
+      if (ml == kMAXLs3 && nl != 16) continue;
       c.push_back({t, ml});
     }
   std::stable_sort(c.begin(), c.end(), [nl](const std::pair<int, int> &a, const std::pair<int, int> &b) {
@@ -1562,6 +1565,12 @@ template <int MODE, int NL>
 int launch_spot_wide(const fdadi_hb *hb, const fdadi_hb::SweepGroup &g, const double *U,
                      const double *Y, double *out, hipStream_t s) {
   const bool long_rows = g.maxl == 2 * kMAXLs;
+  if constexpr (NL == 16) {
+    if (g.maxl == kMAXLs3) {
+      if (g.threads == 256) return launch_spot_t<MODE, 256, NL, kMAXLs3>(hb, g, U, Y, out, s);
+      if (g.threads == 512) return launch_spot_t<MODE, 512, NL, kMAXLs3>(hb, g, U, Y, out, s);
+    }
+  }
   if constexpr (NL != 32) {  // 32-column tiles: 512 threads (16 lanes per line)
     switch (g.threads * 2 + (long_rows ? 1 : 0)) {
       case 512: return launch_spot_t<MODE, 256, NL, kMAXLs>(hb, g, U, Y, out, s);
