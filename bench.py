@@ -534,7 +534,7 @@ def main():
         cfg = {"workload": workload, "queries": nq, "queries_per_step": args.queries_per_step,
                "outside_box_frac": 0.05,
                "labels": sur["labels"], "members": sur["members"],
-               "gate": "error model (regression trees on the device), calibrated at %g failures "
+               "gate": "error model (largest of 4 boosted-tree estimates, on the device), calibrated at %g failures "
                        "above 0.9e-4" % sur["gate_max_fail"],
                "fallback": "batched FP64 ADI, 256x128, nt=200, Craig-Sneyd"}
         steps, warmup = args.steps, args.warmup

@@ -137,6 +137,7 @@ _SIGS = {
     "fdadi_serve_label_inputs": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_i32, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fdadi_serve_scatter": (c_int, [c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "fdadi_serve_max_rows": (c_int, [c_vp, c_i32, c_i64, c_vp, c_vp]),
     "fdadi_gauss_legendre": (c_int, [c_i32, c_vp, c_vp]),
 }
 

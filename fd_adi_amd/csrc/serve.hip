@@ -225,6 +225,17 @@ __global__ void label_inputs_kernel(const double *__restrict__ P, const int32_t 
   }
 }
 
+// out[r] = the largest of the k rows' values S[m * n + r] (the error models'
+// estimates; exact, so host and device agree bit for bit).
+__global__ void max_rows_kernel(const double *__restrict__ S, int k, int64_t n,
+                                double *__restrict__ out) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double v = S[r];
+  for (int m = 1; m < k; ++m) v = fmax(v, S[(int64_t)m * n + r]);
+  out[r] = v;
+}
+
 __global__ void scatter_kernel(const double *__restrict__ src, const int32_t *__restrict__ idx,
                                int m, double *__restrict__ dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -386,6 +397,16 @@ int fdadi_serve_label_inputs(const double *d_P, const int32_t *d_idx, int32_t m,
                 FDADI_ERR_ARG, "fdadi_serve_label_inputs: bad arguments");
   hipLaunchKernelGGL(label_inputs_kernel, dim3(cap), dim3(128), 0, as_stream(stream), d_P, d_idx,
                      m, cap, nt, d_unit, nS, d_params, d_spots, d_K, d_S0, d_v0);
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+int fdadi_serve_max_rows(const double *d_S, int32_t k, int64_t n, double *d_out, void *stream) {
+  FDADI_REQUIRE(k >= 1 && n >= 0 && (n == 0 || (d_S && d_out)), FDADI_ERR_ARG,
+                "fdadi_serve_max_rows: bad arguments");
+  if (n == 0) return FDADI_OK;
+  hipLaunchKernelGGL(max_rows_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream),
+                     d_S, k, n, d_out);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }

@@ -388,6 +388,9 @@ int fdadi_serve_label_inputs(const double *d_P, const int32_t *d_idx, int32_t m,
                              int32_t nt, const double *d_unit, int32_t nS, double *d_params,
                              double *d_spots, double *d_K, double *d_S0, double *d_v0,
                              void *stream);
+/* d_out[r] = max over m < k of d_S[m * n + r]: the largest of k error-model
+ * estimates per row (the gate's score). */
+int fdadi_serve_max_rows(const double *d_S, int32_t k, int64_t n, double *d_out, void *stream);
 /* d_dst[d_idx[i]] = d_src[i], i < m. */
 int fdadi_serve_scatter(const double *d_src, const int32_t *d_idx, int32_t m, double *d_dst,
                         void *stream);
