@@ -420,7 +420,7 @@ class AdiLabeler(object):
     shapes of each engine are picked by ``HestonBatch.autotune``."""
 
     def __init__(self, nS=256, nv=128, nt=200, scheme="cs", theta=0.5,
-                 tiers=(256, 1024, 2048, 4096), tune=True):
+                 tiers=(256, 512, 1024, 2048, 4096), tune=True):
         self.nS, self.nv, self.nt = int(nS), int(nv), int(nt)
         self.scheme, self.theta = scheme, float(theta)
         self.tiers = tuple(sorted(int(t) for t in tiers))
@@ -459,6 +459,26 @@ class AdiLabeler(object):
             if m <= t:
                 return t
         return self.tiers[-1]
+
+    def _batches(self, m):
+        """Batch sizes (tiers) for m rows, in order: full batches of the
+        largest tier, then the rest as the smallest tier that holds it when
+        that pads by at most an eighth of the tier, else the largest tier
+        that fits in it, and so on (padding rows cost as much as real ones)."""
+        caps, rem, big = [], int(m), self.tiers[-1]
+        while rem >= big:
+            caps.append(big)
+            rem -= big
+        while rem > 0:
+            t = self._tier(rem)
+            lower = [x for x in self.tiers if x <= rem]
+            if t - rem <= t // 8 or not lower:
+                caps.append(t)
+                rem = 0
+            else:
+                caps.append(lower[-1])
+                rem -= lower[-1]
+        return caps
 
     def price(self, P, timed=False):
         """FP64 ADI price of every feature row of P (S0 = 100).  With
@@ -517,8 +537,7 @@ class AdiLabeler(object):
             self._unit = SizedArrayPtr(np.ascontiguousarray(_unit_spot_mesh(self.nS)))
         s = _lib.default_stream()
         c0, pending = 0, bool(timed)
-        while c0 < m:
-            cap = self._tier(m - c0)
+        for cap in self._batches(m):
             mm = min(cap, m - c0)
             hb, (K, S0, v0, res) = self._engine(cap)
             ws = self._dev_ws(cap)
