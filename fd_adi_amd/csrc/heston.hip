@@ -2174,7 +2174,9 @@ int fdadi_hb_capture_solve(fdadi_hb *hb, int scheme, double theta, int32_t smoot
   }
   hb->graph = g;
   FDADI_TRY_HIP(hipGraphInstantiate(&hb->gexec, g, nullptr, nullptr, 0));
-  return FDADI_OK;
+  // the capture recorded the refactorisations without running them: run the
+  // last one now, so that eager steps after this call see `scheme` as well
+  return refactor(hb, s);
 }
 
 int fdadi_hb_replay(fdadi_hb *hb, void *stream) {

@@ -116,6 +116,16 @@ def test_whole_solve_graph_with_smoothing(fd):
         eager.sync()
     with pytest.raises(ValueError):
         g.capture_solve("hv", 2, theta=0.6, smoothing_steps=2)
+    # after a capture the engine steps eagerly as configured for the scheme
+    fresh = fd.HestonBatch(P, S, Vs, dts)
+    fresh.capture_solve("hv", 20, theta=0.6, smoothing_steps=2)
+    fresh.set_state(init)
+    fresh.step(5)
+    plain = fd.HestonBatch(P, S, Vs, dts)
+    plain.set_state(init)
+    plain.configure("hv", 0.6)
+    plain.step(5)
+    npt.assert_array_equal(fresh.get_packed(), plain.get_packed())
 
 
 def test_launch_shapes_agree(fd):
