@@ -82,3 +82,20 @@ def test_gate_rejects_everything_without_a_threshold():
     pr = SurrogatePricer(One(), -np.inf, error_model=lambda P, s: np.zeros(len(P)))
     assert not pr.gate(P, np.zeros(len(P))).any()
     assert isinstance(Ensemble([None, None]).members, list)
+
+
+def test_fallback_batches_cover_the_rows_with_little_padding():
+    """AdiLabeler._batches: full batches of the largest tier first, then the
+    remainder in tiers that cover it; padding at most an eighth of the last
+    tier, or at most the smallest tier when nothing smaller fits."""
+    from fd_adi_amd.surrogate import AdiLabeler
+    lab = AdiLabeler(tiers=(256, 512, 1024, 2048, 4096), tune=False)
+    rng = np.random.default_rng(3)
+    for m in list(range(1, 600)) + list(rng.integers(1, 60000, 400)):
+        caps = lab._batches(int(m))
+        assert all(c in lab.tiers for c in caps)
+        assert sum(caps) >= m and sum(caps[:-1]) < m       # every batch holds real rows
+        assert caps[:int(m) // 4096] == [4096] * (int(m) // 4096)
+        waste = sum(caps) - m
+        assert waste <= max(caps[-1] // 8, 255), (m, caps)
+    assert lab._batches(14350) == [4096, 4096, 4096, 2048, 256]
