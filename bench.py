@@ -341,6 +341,26 @@ def probe_labels():
         "5 scale launches on 2^25 doubles done; shapes %s %s" % (hb.get_shape(0), hb.get_shape(1)))
 
 
+def probe_batch():
+    """Fixed config-2 dispatches for kernel traces (see --probe-batch)."""
+    import fd_adi_amd
+    from fd_adi_amd.workloads import batch_draws
+    opts = batch_draws(4096, 2024)
+    hb = fd_adi_amd.HestonBatch([o.params for o in opts], [o.spots for o in opts],
+                                [o.vars for o in opts], [o.dt for o in opts])
+    hb.set_state([o.payoff() for o in opts])
+    hb.configure("cs", 0.5, 1.0)
+    hb.set_shape(0, 512, 16)
+    hb.set_shape(1, 16)
+    hb.step(20)
+    hb.set_shape(0, -1, 16)
+    hb.set_shape(1, -1)
+    hb.step(20)
+    hb.sync()
+    log("probe-batch: 4096 uneven options, 20 CS steps with one launch shape (512, 16) / 16 "
+        "lanes, then 20 with per-option shapes (%d launches per step)" % hb.launches_per_step())
+
+
 def probe_mlp():
     """Fixed MLP dispatch sequence for kernel traces (see --probe-mlp)."""
     import numpy as np
@@ -417,6 +437,9 @@ def main():
     ap.add_argument("--probe-mlp", action="store_true",
                     help="profiling probe: 50 Adam steps of one surrogate network at batch "
                          "8192 on random features and targets; prints no JSON line")
+    ap.add_argument("--probe-batch", action="store_true",
+                    help="profiling probe: the 4096-option uneven batch, 20 CS steps with one "
+                         "launch shape, then 20 with per-option shapes; prints no JSON line")
     ap.add_argument("--probe-cf", action="store_true",
                     help="profiling probe: 5 launches of the semi-analytic Heston pricer on "
                          "2^20 parameter rows; prints no JSON line")
@@ -430,7 +453,8 @@ def main():
     import fd_adi_amd  # noqa: F401
 
     for flag, fn in ((args.probe, probe), (args.probe_labels, probe_labels),
-                     (args.probe_mlp, probe_mlp), (args.probe_cf, probe_cf)):
+                     (args.probe_mlp, probe_mlp), (args.probe_cf, probe_cf),
+                     (args.probe_batch, probe_batch)):
         if flag:
             fn()
             return
