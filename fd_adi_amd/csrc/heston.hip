@@ -2144,6 +2144,9 @@ int fdadi_hb_capture_solve(fdadi_hb *hb, int scheme, double theta, int32_t smoot
     FDADI_TRY_HIP(hipStreamSynchronize(s));
     hb->opts_on_device = true;
   }
+  const int prev_scheme = hb->scheme;
+  const double prev_theta = hb->theta, prev_scale = hb->dt_scale;
+  const bool prev_dev = hb->dev_scalars;
   hb->dev_scalars = false;
   FDADI_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int st = FDADI_OK;
@@ -2164,11 +2167,14 @@ int fdadi_hb_capture_solve(fdadi_hb *hb, int scheme, double theta, int32_t smoot
   for (int32_t n = 0; n < nsteps && st == FDADI_OK; ++n) st = step_once(hb, s);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(s, &g);
-  if (st != FDADI_OK) {
+  if (st != FDADI_OK || e != hipSuccess) {
+    // nothing ran: the engine keeps its previous configuration
     if (g) (void)hipGraphDestroy(g);
-    return st;
-  }
-  if (e != hipSuccess) {
+    hb->scheme = prev_scheme;
+    hb->theta = prev_theta;
+    hb->dt_scale = prev_scale;
+    hb->dev_scalars = prev_dev;
+    if (st != FDADI_OK) return st;
     set_error(std::string("fdadi_hb_capture_solve: ") + hipGetErrorString(e));
     return FDADI_ERR_HIP;
   }
