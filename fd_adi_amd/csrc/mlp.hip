@@ -94,13 +94,23 @@ constexpr int kDepth = 2;
 constexpr int kKBlock = kKStep * kDepth;  // K granularity of the GEMMs
 constexpr int kChunks = kTile * kKStep / 8 / kThreads;  // 16-byte loads per operand per stage
 
+// One GEMM of a launch: operands, sizes, epilogue and its tile grid (gx x gy
+// output tiles, gz splits of K).
+struct GemmArgs {
+  const bf16 *A, *B;
+  int M, N, K, gx, gy, gz;
+  EpiArgs ep;
+};
+
+// The work of work-group `Lin` (linear index within its GEMM's grid).
 template <int EPI>
-__global__ void __launch_bounds__(kThreads)
-gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, int N, int K,
-               EpiArgs ep) {
+__device__ __forceinline__ void gemm_tile(bf16 *smem, const GemmArgs &g, int Lin) {
+  const bf16 *__restrict__ A = g.A;
+  const bf16 *__restrict__ B = g.B;
+  const int M = g.M, N = g.N, K = g.K;
+  const EpiArgs &ep = g.ep;
   // Two LDS stages: the MFMAs of stage s read buffer s % 2 while stage s + 1
   // is written to the other one, so one barrier per stage suffices.
-  __shared__ __attribute__((aligned(16))) bf16 smem[gemm_lds(EPI)];
   bf16 *const As[2] = {smem, smem + kTile * kLdsPitch};
   bf16 *const Bs[2] = {smem + 2 * kTile * kLdsPitch, smem + 3 * kTile * kLdsPitch};
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
@@ -108,13 +118,13 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
   // Work-groups are dealt to the 8 XCDs round-robin by linear id; renumber
   // them so that each XCD gets a contiguous run of tiles (x fastest): the
   // N tiles of one row block of A, and of one split, then share that XCD's L2.
-  const int gx = gridDim.x, gy = gridDim.y, total = gx * gy * (int)gridDim.z;
-  int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int gx = g.gx, gy = g.gy, total = gx * gy * g.gz;
+  int L = Lin;
   if (total % 8 == 0) L = (L % 8) * (total / 8) + L / 8;
   const int bx = L % gx, by = (L / gx) % gy, bz = L / (gx * gy);
   const int m0 = by * kTile, n0 = bx * kTile;
-  // split-K: work-group z covers K columns [z*Ks, (z+1)*Ks) (gridDim.z = 1: all)
-  const int Ks = K / (int)gridDim.z, kbeg = bz * Ks, nsteps = Ks / kKStep;
+  // split-K: work-group z covers K columns [z*Ks, (z+1)*Ks) (gz = 1: all)
+  const int Ks = K / g.gz, kbeg = bz * Ks, nsteps = Ks / kKStep;
   f32x4 acc[kBM][kBN];
 #pragma unroll
   for (int a = 0; a < kBM; ++a)
@@ -291,6 +301,26 @@ gemm_nt_kernel(const bf16 *__restrict__ A, const bf16 *__restrict__ B, int M, in
   if (EPI == EPI_FWD) tile_to_global(DT, ep.dT, M, n0, m0);
 }
 
+template <int EPI>
+__global__ void __launch_bounds__(kThreads) gemm_nt_kernel(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 smem[gemm_lds(EPI)];
+  gemm_tile<EPI>(smem, g, blockIdx.x + g.gx * (blockIdx.y + g.gy * blockIdx.z));
+}
+
+// Two independent GEMMs in one launch (work-groups [0, n1) the first, the
+// rest the second), so that their work-groups share the CUs: a layer's
+// backward-data and weight-gradient products.  n1 is a multiple of 8, so
+// each GEMM's work-groups keep their XCD order.
+template <int E1, int E2>
+__global__ void __launch_bounds__(kThreads) gemm_pair_kernel(GemmArgs g1, GemmArgs g2, int n1) {
+  constexpr int kL1 = gemm_lds(E1), kL2 = gemm_lds(E2);
+  __shared__ __attribute__((aligned(16))) bf16 smem[kL1 > kL2 ? kL1 : kL2];
+  if ((int)blockIdx.x < n1)
+    gemm_tile<E1>(smem, g1, blockIdx.x);
+  else
+    gemm_tile<E2>(smem, g2, blockIdx.x - n1);
+}
+
 // Features -> padded, normalised bf16 input rows:
 // x_f = 2 (p_f - lo_f) / (hi_f - lo_f) - 1 for the nin features of a row,
 // 0 for the padding; one thread per element, features fastest (coalesced).
@@ -465,6 +495,7 @@ layer_grad_kernel(const float *__restrict__ part, int splits, int64_t n, float *
     const int64_t i = 4 * (blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
     if (i >= n) return;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
     for (int z = 0; z < splits; ++z)
       acc += *reinterpret_cast<const f32x4 *>(part + (size_t)z * n + i);
     *reinterpret_cast<f32x4 *>(out + i) = acc;
@@ -576,19 +607,38 @@ namespace {
 
 int in_dim(int l) { return l == 0 ? kInPad : kH; }
 
-int gemm(int epi, const bf16 *A, const bf16 *B, int M, int N, int K, const EpiArgs &ep,
-         hipStream_t s, int splits = 1) {
+int gemm_args(int epi, const bf16 *A, const bf16 *B, int M, int N, int K, const EpiArgs &ep,
+              int splits, GemmArgs *g) {
   FDADI_REQUIRE(M % kTile == 0 && N % kTile == 0 && K % kKBlock == 0, FDADI_ERR_ARG,
                 "mlp gemm: dimensions must be multiples of 128 x 128 x 128");
   FDADI_REQUIRE(splits >= 1 && (splits == 1 || epi == EPI_F32) && K % (kKBlock * splits) == 0,
                 FDADI_ERR_ARG, "mlp gemm: split-K needs the fp32 epilogue and K % (128 s) == 0");
-  dim3 grid(N / kTile, M / kTile, splits);
+  *g = GemmArgs{A, B, M, N, K, N / kTile, M / kTile, splits, ep};
+  return FDADI_OK;
+}
+
+int gemm(int epi, const bf16 *A, const bf16 *B, int M, int N, int K, const EpiArgs &ep,
+         hipStream_t s, int splits = 1) {
+  GemmArgs g;
+  FDADI_RET(gemm_args(epi, A, B, M, N, K, ep, splits, &g));
+  dim3 grid(g.gx, g.gy, g.gz);
   switch (epi) {
-    case EPI_FWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
-    case EPI_INF: hipLaunchKernelGGL(gemm_nt_kernel<EPI_INF>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
-    case EPI_BWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
-    default: hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, dim3(kThreads), 0, s, A, B, M, N, K, ep); break;
+    case EPI_FWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, dim3(kThreads), 0, s, g); break;
+    case EPI_INF: hipLaunchKernelGGL(gemm_nt_kernel<EPI_INF>, grid, dim3(kThreads), 0, s, g); break;
+    case EPI_BWD: hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, dim3(kThreads), 0, s, g); break;
+    default: hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, dim3(kThreads), 0, s, g); break;
   }
+  FDADI_CHECK_LAUNCH();
+  return FDADI_OK;
+}
+
+// A weight-gradient GEMM (fp32, split-K) and a backward-data GEMM in one
+// launch.
+int gemm_pair_f32_bwd(const GemmArgs &w, const GemmArgs &b, hipStream_t s) {
+  const int n1 = w.gx * w.gy * w.gz, n2 = b.gx * b.gy * b.gz;
+  FDADI_REQUIRE(n1 % 8 == 0, FDADI_ERR_ARG, "mlp gemm pair: first grid not a multiple of 8");
+  hipLaunchKernelGGL((gemm_pair_kernel<EPI_F32, EPI_BWD>), dim3(n1 + n2), dim3(kThreads), 0, s, w,
+                     b, n1);
   FDADI_CHECK_LAUNCH();
   return FDADI_OK;
 }
@@ -885,19 +935,28 @@ static int grad_impl(fdadi_mlp *m, const double *d_P, const double *d_T, const i
       splits *= 2;
     EpiArgs ew{};
     ew.C = splits > 1 ? m->wpart : m->g + m->off_W[l];
-    FDADI_RET(gemm(EPI_F32, dzt, aT, kH, in_dim(l), Mpad, ew, s, splits));
+    GemmArgs gw;
+    FDADI_RET(gemm_args(EPI_F32, dzt, aT, kH, in_dim(l), Mpad, ew, splits, &gw));
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}), independent of dW_l: both in
+      // one launch (the weight-gradient grid, 16 tiles x splits, is a
+      // multiple of 8)
+      EpiArgs eb{};
+      eb.dT = m->dT[l - 1];
+      eb.out = dz2;
+      eb.outT = dzt2;
+      GemmArgs gb;
+      FDADI_RET(gemm_args(EPI_BWD, dz, m->WbT[l], Mpad, kH, kH, eb, 1, &gb));
+      FDADI_RET(gemm_pair_f32_bwd(gw, gb, s));
+    } else {
+      FDADI_RET(gemm(EPI_F32, dzt, aT, kH, in_dim(l), Mpad, ew, s, splits));
+    }
     const int64_t nw = (int64_t)kH * in_dim(l);
     const int nsum = splits > 1 ? (int)grid_for(nw / 4, 256) : 0;
     hipLaunchKernelGGL(layer_grad_kernel, dim3(nsum + kH), dim3(256), 0, s, m->wpart, splits, nw,
                        m->g + m->off_W[l], nsum, dzt, Mpad, m->g + m->off_b[l]);
     FDADI_CHECK_LAUNCH();
     if (l > 0) {
-      // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1})
-      EpiArgs eb{};
-      eb.dT = m->dT[l - 1];
-      eb.out = dz2;
-      eb.outT = dzt2;
-      FDADI_RET(gemm(EPI_BWD, dz, m->WbT[l], Mpad, kH, kH, eb, s));
       std::swap(dz, dz2);
       std::swap(dzt, dzt2);
     }
