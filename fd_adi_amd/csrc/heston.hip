@@ -374,6 +374,22 @@ __device__ __forceinline__ Raw2 load_raw2(const double *__restrict__ f, unsigned
   return Raw2{p[lc.jx], p[lc.jc]};
 }
 
+// A double at a 32-bit byte offset from a work-group-uniform base: the global
+// access takes the scalar base + 32-bit lane offset form, with no 64-bit
+// address arithmetic per lane.
+__device__ __forceinline__ double load_at(const double *__restrict__ base, unsigned byte_off) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+__device__ __forceinline__ void store_at(double *__restrict__ base, unsigned byte_off, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
+// load_raw2 with the row given as a byte offset (row * nv * 8).
+__device__ __forceinline__ Raw2 load_raw2_at(const double *__restrict__ f, unsigned rowb,
+                                             const LaneCols &lc) {
+  return Raw2{load_at(f, rowb + lc.jx * 8u), load_at(f, rowb + lc.jc * 8u)};
+}
+
 // Value of the lane before / after / two before within the tile's lanes
 // (quad_perm for four columns; row_shr / row_shl within a DPP row of 16
 // lanes for 8 and 16; wave_shr / wave_shl across the two DPP rows of a
@@ -401,11 +417,29 @@ __device__ __forceinline__ double from_prev2_col(double x) {
   return dpp_mov<NL == 4 ? 0x40 : 0x112>(x);  // quad_perm [0,0,0,1] / row_shr:2
 }
 
+// The same move with a fallback: lanes whose source lane lies outside their
+// DPP row (row_shr:1 on the row's first lane, row_shl:1 on its last) keep
+// `old` (bound_ctrl off).
+template <int CTRL>
+__device__ __forceinline__ double dpp_or(double old, double x) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(x), CTRL, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(x), CTRL, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 template <int NL>
 __device__ __forceinline__ Row4 expand_raw(const Raw2 &w, int jj, bool last) {
-  const double l = from_prev_col<NL>(w.c), r = from_next_col<NL>(w.c);
   const double m2 = from_prev2_col<NL>(w.c);
-  return Row4{jj == 0 ? w.e : l, w.c, jj == NL - 1 ? w.e : r, last ? m2 : 0.0};
+  if constexpr (NL == 16) {
+    // a DPP row of 16 lanes is the tile: its first and last lanes, which have
+    // no source lane, take the edge load
+    return Row4{dpp_or<0x111>(w.e, w.c), w.c, dpp_or<0x101>(w.e, w.c), last ? m2 : 0.0};
+  } else {
+    const double l = from_prev_col<NL>(w.c), r = from_next_col<NL>(w.c);
+    return Row4{jj == 0 ? w.e : l, w.c, jj == NL - 1 ? w.e : r, last ? m2 : 0.0};
+  }
 }
 template <int NL>
 __device__ __forceinline__ Row4 expand_raw(const Row4 &w, int, bool) {
@@ -421,6 +455,10 @@ struct RawRow {
                                               const LaneCols &lc) {
     return load_raw2(f, rowoff, lc);
   }
+  static __device__ __forceinline__ Raw2 load_b(const double *__restrict__ f, unsigned rowb,
+                                                const LaneCols &lc) {
+    return load_raw2_at(f, rowb, lc);
+  }
   static __device__ __forceinline__ Raw2 zero() { return Raw2{0.0, 0.0}; }
 };
 template <>
@@ -429,6 +467,10 @@ struct RawRow<false> {
   static __device__ __forceinline__ Row4 load(const double *__restrict__ f, unsigned rowoff,
                                               const LaneCols &lc) {
     return load_row4(f, rowoff, lc);
+  }
+  static __device__ __forceinline__ Row4 load_b(const double *__restrict__ f, unsigned rowb,
+                                                const LaneCols &lc) {
+    return load_row4(f, rowb / 8u, lc);
   }
   static __device__ __forceinline__ Row4 zero() { return Row4{0.0, 0.0, 0.0, 0.0}; }
 };
@@ -685,6 +727,10 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   }
   if constexpr (TAB_LDS) __syncthreads();
   const double *tch = tab + ch * CHP;
+  // byte offset of the row the walk loads next (row s + 1 + PF + k at step k),
+  // clamped to the last row; nS <= 2048 and nv <= 1024 keep it far below 2^32
+  const unsigned rstep = unv * 8u, rlast = (unsigned)(nS - 1) * rstep;
+  unsigned rnext = (unsigned)(s + 1 + PF) * rstep;
 
   double mu[MAXL], nreg[MAXL], dv[MAXL];
 #pragma unroll
@@ -705,9 +751,10 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
       Row4 yn{0.0, 0.0, 0.0, 0.0};
       if constexpr (HAS_Y) yn = expand_raw<NL>(ry[slot], jj, last);
       enter_row<MODE>(q, un, yn, wc);
-      const unsigned rr = clamp_row(i + 1 + PF) * unv;
-      ru[slot] = RR::load(u, rr, lc);
-      if constexpr (HAS_Y) ry[slot] = RR::load(y2, rr, lc);
+      const unsigned rb = rnext < rlast ? rnext : rlast;
+      rnext += rstep;
+      ru[slot] = RR::load_b(u, rb, lc);
+      if constexpr (HAS_Y) ry[slot] = RR::load_b(y2, rb, lc);
     }
     if constexpr (!TAB_LDS) R = quad_row_fields(ta0, tb0);
 
@@ -844,13 +891,15 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
   // Backward fix-up; stage output x - tdt*A1(f) written straight from the chunk.
   double *ob = out + o.off;
   double x = cIn[jj * (TPL + 1) + ch];
-  const unsigned uj = (unsigned)j;
+  // byte offset of (row s + k, column j), stepped down the chunk
+  unsigned sb = (unsigned)(s + MAXL - 1) * rstep + (unsigned)j * 8u;
   if (full) {
 #pragma unroll
     for (int k = MAXL - 1; k >= 0; --k) {
       const int i = s + k;
       x = dv[k] + nreg[k] * x;
-      if (col_ok && i < nS) ob[(unsigned)i * unv + uj] = HAS_EX ? x - eb[k * THREADS + tid] : x;
+      if (col_ok && i < nS) store_at(ob, sb, HAS_EX ? x - eb[k * THREADS + tid] : x);
+      sb -= rstep;
     }
   } else {
 #pragma unroll
@@ -858,8 +907,9 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
       const int i = s + k;
       if (k < L) {
         x = dv[k] + nreg[k] * x;
-        if (col_ok && i < nS) ob[(unsigned)i * unv + uj] = HAS_EX ? x - eb[k * THREADS + tid] : x;
+        if (col_ok && i < nS) store_at(ob, sb, HAS_EX ? x - eb[k * THREADS + tid] : x);
       }
+      sb -= rstep;
     }
   }
 }
