@@ -417,25 +417,13 @@ __device__ __forceinline__ double from_prev2_col(double x) {
   return dpp_mov<NL == 4 ? 0x40 : 0x112>(x);  // quad_perm [0,0,0,1] / row_shr:2
 }
 
-// The same move with a fallback: lanes whose source lane lies outside their
-// DPP row (row_shr:1 on the row's first lane, row_shl:1 on its last) keep
-// `old` (bound_ctrl off).
-template <int CTRL>
-__device__ __forceinline__ double dpp_or(double old, double x) {
-  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(x), CTRL, 0xF,
-                                             0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(x), CTRL, 0xF,
-                                             0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
 template <int NL>
 __device__ __forceinline__ Row4 expand_raw(const Raw2 &w, int jj, bool last) {
   const double m2 = from_prev2_col<NL>(w.c);
   if constexpr (NL == 16) {
     // a DPP row of 16 lanes is the tile: its first and last lanes, which have
-    // no source lane, take the edge load
-    return Row4{dpp_or<0x111>(w.e, w.c), w.c, dpp_or<0x101>(w.e, w.c), last ? m2 : 0.0};
+    // no source lane for row_shr:1 / row_shl:1, keep the edge load
+    return Row4{dpp_or<0x111, 0xF>(w.c, w.e), w.c, dpp_or<0x101, 0xF>(w.c, w.e), last ? m2 : 0.0};
   } else {
     const double l = from_prev_col<NL>(w.c), r = from_next_col<NL>(w.c);
     return Row4{jj == 0 ? w.e : l, w.c, jj == NL - 1 ? w.e : r, last ? m2 : 0.0};
