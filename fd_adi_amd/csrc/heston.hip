@@ -798,8 +798,11 @@ spot_sweep_kernel(Ctx c, const Tile *__restrict__ tiles, const double *__restric
     // the solve forces Dirichlet rows (0, and nS-1 under a barrier) to 0
     if ((k == 0 && s == 0) || (barrier && i == nS - 1)) rhs = 0.0;
     // I - tdt*A0 on this row (identity rows on Dirichlet rows and zero
-    // off-diagonals at the line ends come from the bands), scaled by the pivot
-    const double rd0 = 1.0 / ((1.0 - tmid) - (-tsub * rdp) * cp);
+    // off-diagonals at the line ends come from the bands), scaled by the
+    // pivot.  The diagonal 1 - tdt*b is one fused operation, as the
+    // factorisation forms it (li0_row), written out so that the compiler's
+    // contraction choices cannot change how the recomputed pivots round.
+    const double rd0 = 1.0 / (__builtin_fma(-tdt, B.mid, 1.0) - (-tsub * rdp) * cp);
     rdp = rd0;
     cp = -tsup;
     const double dsc = rd0 * rhs;
