@@ -80,6 +80,25 @@ def test_labeler_batches_tails_and_graph_reuse(S, labeler):
     npt.assert_allclose(a[[0, 17, 36]], c, rtol=1e-13, atol=0)
 
 
+def test_labels_reproduce_bit_for_bit_at_one_launch_shape(S):
+    """Fresh labellers (new engines, graphs and buffers) with the same launch
+    shape give bit-identical labels, and per-option shapes that resolve to
+    the same tiles (every row here is 256 x 128) give the same bits: the
+    training data depends on the launch shape only, not on the run."""
+    P = S.draw_params(64, 11)
+    out = []
+    for shape in [(256, 16), (256, 16), (-1, 16)]:
+        lab = S.AdiLabeler(NS, NV, NT, tiers=(64,), tune=False)
+        hb, _ = lab._engine(64)
+        hb.set_shape(0, *shape)
+        hb.set_shape(1, 8)
+        hb.capture(NT)
+        out.append(lab.price(P))
+    assert np.all(np.isfinite(out[0]))
+    npt.assert_array_equal(out[0], out[1])
+    npt.assert_array_equal(out[0], out[2])
+
+
 def test_network_gains_over_the_semi_analytic_price(S, labeler):
     """What the trained network adds: on held-out rows, the semi-analytic
     price plus the network's correction is closer to the FP64 ADI label than
