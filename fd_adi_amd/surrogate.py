@@ -731,6 +731,25 @@ def calibrate_threshold(score, err, atol=1e-4, max_fail=0.0):
     return float(s[k]), (k + 1) / float(s.size)
 
 
+def calibration_spread(score, err, atol=1e-4, max_fail=0.0, n=200, seed=0,
+                       q=(0.05, 0.5, 0.95)):
+    """Quantiles q of the accepted fraction that ``calibrate_threshold``
+    gives over n bootstrap resamples (rows drawn with replacement) of the
+    calibration rows: how far the admission could move with another draw
+    of those rows.  A zero-failure threshold is set by the lowest-scored
+    failing rows, so this spread is wide where few failures lie near it."""
+    score = np.asarray(score, dtype=float)
+    err = np.asarray(err, dtype=float)
+    if score.size == 0:
+        return {"%g" % a: 0.0 for a in q}
+    rng = np.random.default_rng(seed)
+    acc = np.empty(n)
+    for b in range(n):
+        i = rng.integers(0, score.size, score.size)
+        acc[b] = calibrate_threshold(score[i], err[i], atol, max_fail)[1]
+    return {"%g" % a: float(v) for a, v in zip(q, np.quantile(acc, q))}
+
+
 def calibrate_gate(ensemble, P, y, atol=1e-4, feller_min=0.0, max_fail=0.0):
     """Spread threshold of an ensemble (at least two members) calibrated on
     held-out rows P with FP64 ADI labels y (``calibrate_threshold``); rows
@@ -868,6 +887,9 @@ def fit_surrogate(P, y, members=4, steps=16000, batch=8192, fmap="identity", ato
                 heldout_frac_within={"%g" % a: float(np.mean(err <= a))
                                      for a in (1e-5, 1e-4, 1e-3)},
                 calib_accept=float(frac * region.mean()), tau=float(tau),
+                calib_accept_bootstrap={
+                    k: v * float(region.mean()) for k, v in calibration_spread(
+                        score[region], err[nfit:][region], catol, max_fail).items()},
                 train_final_loss=[float(h[-1][1]) for h in hist if h])
     return SurrogatePricer(ens, tau, feller_min=feller_min, error_model=em), info
 

@@ -65,6 +65,26 @@ def test_calibrate_threshold_ties_and_empty():
     assert calibrate_threshold([], [], atol=0.5) == (-np.inf, 0.0)
 
 
+def test_calibration_spread_of_one_failing_row():
+    """One failing row among N, scored 300th lowest: a resample holds it with
+    probability 1 - (1 - 1/N)^N = 0.632, and then admits the rows below it
+    (about 0.30 of the resample); otherwise nothing fails and everything is
+    admitted.  So the median sits near 0.30 and the 95th percentile at 1."""
+    from fd_adi_amd.surrogate import calibration_spread
+    N = 1000
+    s = np.arange(N, dtype=float)
+    e = np.zeros(N)
+    e[300] = 1.0
+    q = calibration_spread(s, e, atol=0.5, n=400, seed=3)
+    assert abs(q["0.5"] - 0.30) < 0.04
+    assert q["0.95"] == 1.0
+    assert 0.2 < q["0.05"] <= q["0.5"]
+    # no failing row: every resample admits everything
+    q = calibration_spread(s, np.zeros(N), atol=0.5, n=50)
+    assert q == {"0.05": 1.0, "0.5": 1.0, "0.95": 1.0}
+    assert calibration_spread([], [], n=5) == {"0.05": 0.0, "0.5": 0.0, "0.95": 0.0}
+
+
 def test_gate_rejects_everything_without_a_threshold():
     from fd_adi_amd.surrogate import SurrogatePricer, Ensemble
     P = draw_params(8, 1)
